@@ -1,0 +1,280 @@
+"""Benchmark: visibilities/s of the per-time computation on one MI355X.
+
+Workload (north-star headline): 350 antennas on a 14.6 m hexagonal grid with
++-2 cm jitter, 100,000 point sources (RA uniform, sin(dec) uniform on
+[-90, +30] deg, log-normal(0, 1) Jy fluxes), polarized (nfeed = nax = 2),
+20 frequencies over 100-200 MHz, times 2 min apart, one shared complex64
+E-field beam on a 360 x 91 (1 deg) az/za grid with bicubic interpolation.
+Synthetic, seeded, generated here; inputs are resident in HBM before timing.
+
+A step is one time sample: rotation + cull once, then for each of the 20
+frequencies the beam, the fused phase/Z kernel, the HERK and the expansion
+into the API layout, results left in device memory.  Two forms are timed:
+split-BF16 (the headline `value`; Z = hi + lo bfloat16, four term products
+on BF16 MFMA with FP32 accumulation, FP64 cross-slice sums) and plain FP64.
+
+Prints one JSON line.  `--gpus N` with N > 1 exits: this build targets one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+NANT = 350
+NSRC = 100_000
+NFREQ = 20
+NTIMES_POOL = 30
+MFMA_BF16_PEAK = 2516.6e12
+FP64_PEAK = 78.6e12
+
+
+def build_workload(nsrc=NSRC, nant=NANT, nfreq=NFREQ, seed=1):
+    from matvis_amd import astro, synth
+
+    antpos = synth.hex_array(nant, 14.6, 0.02, seed=seed)
+    ra, dec = synth.random_sky(nsrc, seed + 1, -90.0, 30.0)
+    flux0 = synth.lognormal_fluxes(nsrc, seed + 2, 0.0, 1.0)
+    freqs = np.linspace(100e6, 200e6, nfreq)
+    fluxes = np.repeat(flux0[:, None], nfreq, 1)
+    times = synth.times_spaced(NTIMES_POOL, 2.0)
+    loc = synth.hera_location()
+    beam = synth.harmonic_efield_beam(seed=seed + 3, naz=360, nza=91)
+    rot = astro.rotation_matrices(times, loc)
+    return dict(antpos=antpos, ra=ra, dec=dec, fluxes=fluxes, freqs=freqs,
+                times=times, loc=loc, beam=beam, rotations=rot)
+
+
+def make_engine(w, form, nterms=1, nprod=1, precision=2):
+    from matvis_amd import astro
+    from matvis_amd.core.beams import _wrangle_beams
+    from matvis_amd.gpu import coords as gcoords
+    from matvis_amd.gpu.engine import VisibilityEngine
+
+    nsrc = w["ra"].size
+    co = gcoords.CoordinateRotationMatrix(
+        rotations=w["rotations"], flux=np.sqrt(0.5 * w["fluxes"][:, 0]),
+        times=w["times"], telescope_loc=w["loc"],
+        skycoords=astro.SkyCoord(w["ra"], w["dec"]), chunk_size=nsrc,
+        precision=precision, source_buffer=1.0)
+    co.setup()
+    beams, _, _ = _wrangle_beams(None, [w["beam"]], True, len(w["antpos"]),
+                                 float(w["freqs"][0]))
+    eng = VisibilityEngine(
+        antpos=w["antpos"], freqs=w["freqs"], coords=co, fluxes=w["fluxes"],
+        beam_list=beams, beam_idx=None, polarized=True, precision=precision,
+        form=form, nterms=nterms, nprod=nprod, order=3, nchunks=1,
+        chunk_size=nsrc, nsrc_alloc=nsrc)
+    eng.setup()
+    return eng
+
+
+class HerkTimer:
+    """HIP events around every HERK launch, on the launching stream."""
+
+    def __init__(self):
+        import torch
+
+        self.torch = torch
+        self.pairs = []
+        self.enabled = False
+
+    def before(self, stream):
+        if self.enabled:
+            e = self.torch.cuda.Event(enable_timing=True)
+            e.record(stream)
+            self.pairs.append([e, None])
+
+    def after(self, stream):
+        if self.enabled:
+            e = self.torch.cuda.Event(enable_timing=True)
+            e.record(stream)
+            self.pairs[-1][1] = e
+
+    def mean_ms(self):
+        return float(np.mean([a.elapsed_time(b) for a, b in self.pairs]))
+
+
+def time_form(eng, steps, warmup, timer=None):
+    """Time `steps` time samples; returns (seconds, per-step ms, n_up list)."""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    out = eng.new_output()
+    for i in range(warmup):
+        eng.compute_time(i % NTIMES_POOL, out, stream=stream)
+    torch.cuda.synchronize()
+    if timer is not None:
+        eng.herk_hooks = (timer.before, timer.after)
+        timer.enabled = True
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    nups = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs[0].record(stream)
+    for i in range(steps):
+        t = (warmup + i) % NTIMES_POOL
+        eng.compute_time(t, out, stream=stream)
+        evs[i + 1].record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    if timer is not None:
+        timer.enabled = False
+        eng.herk_hooks = None
+    per_step = [evs[i].elapsed_time(evs[i + 1]) for i in range(steps)]
+    for i in range(steps):
+        t = (warmup + i) % NTIMES_POOL
+        rot = eng.coords._rots[t]
+        nups.append(int(((rot[2] @ eng.coords_eci_host) > 0).sum()))
+    return elapsed, per_step, nups
+
+
+def cpu_baseline(w, ninst_freqs=2):
+    """The oracle on one time x `ninst_freqs` frequencies, all host cores."""
+    import oracle as O
+
+    cores = os.cpu_count() or 1
+    eci = O.point_source_crd_eq(w["ra"], w["dec"])
+    from matvis_amd.beams import grid_planes
+
+    planes, daz, dza, azmin, _ = grid_planes(w["beam"], 150e6, True)
+    beam = {"kind": "grid", "planes": planes[None], "daz": [daz],
+            "dza": [dza], "azmin": [azmin], "order": 3}
+    t0 = time.perf_counter()
+    outs = []
+    for f in range(ninst_freqs):
+        outs.append(O.simulate(antpos=w["antpos"], freq=w["freqs"][f], eci=eci,
+                               rotations=w["rotations"][:1],
+                               I_sky=w["fluxes"][:, f], beam=beam,
+                               polarized=True))
+    dt = time.perf_counter() - t0
+    vis = ninst_freqs * NANT * NANT * 4
+    return vis / dt, cores, outs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nsrc", type=int, default=NSRC)
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline (for profiling runs)")
+    ap.add_argument("--forms", default="bf16,fp64")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(instances shard by (time, frequency); see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from matvis_amd._lib import FORM_C128, FORM_SPLIT_BF16
+    from matvis_amd.coordinates import point_source_crd_eq
+
+    torch.cuda.set_device(0)
+    w = build_workload(nsrc=args.nsrc)
+    eci_h = point_source_crd_eq(w["ra"], w["dec"])
+    forms = args.forms.split(",")
+    results = {}
+    outs = {}
+    vis_per_inst = NANT * NANT * 4
+    m = 2 * NANT
+    for name in forms:
+        form, nt, npd = ((FORM_SPLIT_BF16, 2, 4) if name == "bf16"
+                         else (FORM_C128, 1, 1))
+        eng = make_engine(w, form, nt, npd)
+        eng.coords_eci_host = eci_h
+        timer = HerkTimer()
+        steps = args.steps
+        elapsed, per_step, nups = time_form(eng, steps, args.warmup, timer)
+        herk_ms = timer.mean_ms()
+        k_mean = 2 * float(np.mean(nups))
+        alg_flops = 8.0 * k_mean * m * (m + 1) / 2
+        inst = steps * NFREQ
+        results[name] = dict(
+            seconds=elapsed, ms_per_step=1e3 * elapsed / steps,
+            vis_per_s=inst * vis_per_inst / elapsed,
+            src_bl_per_s=NFREQ * sum(n * NANT * (NANT + 1) / 2 for n in nups)
+            / elapsed,
+            p50_ms=float(np.percentile(per_step, 50)),
+            p99_ms=float(np.percentile(per_step, 99)),
+            herk_ms=herk_ms, herk_alg_flops=alg_flops,
+            herk_tflops=alg_flops / herk_ms / 1e9, n_up_mean=float(np.mean(nups)),
+        )
+        # keep one instance (time 0, first frequencies) for the error figures
+        out = eng.new_output()
+        eng.compute_time(0, out)
+        torch.cuda.synchronize()
+        outs[name] = out[:2].cpu().numpy()
+        del eng
+        torch.cuda.empty_cache()
+
+    def nerr(a, b):
+        import oracle as O
+        return O.normalized_error(a, b, NANT, 2)
+
+    err = {}
+    if "bf16" in outs and "fp64" in outs:
+        err["bf16_vs_fp64_gpu"] = nerr(outs["bf16"], outs["fp64"])
+    cpu = None
+    if not args.no_cpu:
+        cpu_vis, cores, ref = cpu_baseline(w, 2)
+        ref = np.stack([r[0] for r in ref])
+        for name in outs:
+            err[f"{name}_vs_oracle"] = nerr(outs[name], ref)
+        cpu = {"value": cpu_vis, "unit": "vis/s", "cores": cores,
+               "kind": "port",
+               "sample": "time 0 x 2 frequencies (100, 105.3 MHz) of the same "
+                         "workload: 2 instances, numpy/OpenBLAS ZGEMM over "
+                         "the padded source axis"}
+
+    head = results.get("bf16") or results[forms[0]]
+    herk_bound = "mfma"
+    peak = MFMA_BF16_PEAK if "bf16" in results else FP64_PEAK
+    achieved = head["herk_tflops"]
+    line = {
+        "metric": "vis/s at 350 ant x 100k src pol, split-BF16 vs FP64 GPU "
+                  "@<=1e-6 rel err; p50/p99",
+        "value": head["vis_per_s"],
+        "unit": "vis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if "bf16" in results else "f64",
+        "data": "synthetic (seeded hex array, random sky, harmonic E-field beam)",
+        "config": {"workload": "350 antennas x 100k sources, polarized, 20 "
+                               "freqs, bicubic c64 beam; step = 1 time x 20 "
+                               "freqs", "nant": NANT, "nsrc": args.nsrc,
+                   "nfreq": NFREQ, "form": "split-bf16 (2 terms, 4 products)"},
+        "p50_ms": head["p50_ms"],
+        "p99_ms": head["p99_ms"],
+        "roofline": {"bound": herk_bound, "achieved": achieved,
+                     "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / (peak / 1e12), "traffic": None},
+        "cpu_baseline": cpu,
+        "forms": results,
+        "errors_normalised_max_median_elemmax_elemmedian": err,
+    }
+    if "fp64" in results and "bf16" in results:
+        line["speedup_bf16_over_fp64"] = (results["bf16"]["vis_per_s"]
+                                          / results["fp64"]["vis_per_s"])
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

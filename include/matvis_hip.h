@@ -1,0 +1,230 @@
+/*
+ * matvis_hip.h -- C ABI of libmatvis_hip.so, the MI355X (gfx950) kernels behind
+ * matvis's per-time visibility computation.
+ *
+ * The reference boundary is Python only: matvis.gpu.simulate
+ * (src/matvis/gpu/gpu.py:57-256) drives five plugin classes whose device work
+ * is CuPy/cuBLAS.  Each entry point below replaces one of those device calls;
+ * the comment on each names the reference code it stands in for.  The Python
+ * package matvis_amd binds these through ctypes (matvis_amd/_lib.py).
+ *
+ * Conventions
+ *  - Every entry point returns an int status: MV_OK (0) or a negative code.
+ *    mv_last_error() returns the message of the last failure on this thread.
+ *  - All array arguments are device pointers owned by the caller, except where
+ *    a parameter is documented as host memory.  Complex arrays are interleaved
+ *    (re, im) pairs of double (c128) or float (c64).
+ *  - `stream` is a hipStream_t passed as an opaque pointer (NULL = the null
+ *    stream).  Work is enqueued on it; nothing synchronizes unless stated.
+ *  - Counts that are only known on the device (sources above the horizon) are
+ *    passed as a device pointer `n_dev`; when it is NULL the host count is
+ *    used, and otherwise the host count bounds it (min of the two), so a
+ *    device count larger than the buffers never indexes past them.
+ *  - Nothing here allocates device memory: scratch space is a caller-provided
+ *    workspace whose size a *_workspace_bytes() query returns.
+ */
+#ifndef MATVIS_HIP_H
+#define MATVIS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *mv_stream_t;
+
+#define MV_OK 0
+#define MV_ERR_ARG (-1)      /* invalid argument (shape, pointer, enum)     */
+#define MV_ERR_HIP (-2)      /* a HIP runtime call or launch failed          */
+#define MV_ERR_CAPACITY (-3) /* a size exceeds what a buffer can hold        */
+
+/* Z / HERK numeric forms. */
+#define MV_FORM_C128 0       /* plain complex128, FP64 MFMA                  */
+#define MV_FORM_C64 1        /* plain complex64, FP32 MFMA                   */
+#define MV_FORM_SPLIT_BF16 2 /* Z split into bf16 terms, BF16 MFMA + FP32 acc */
+#define MV_FORM_SPLIT_F16 3  /* Z split into fp16 terms, FP16 MFMA + FP32 acc */
+
+/* Copies the last error message of the calling thread into buf (truncated,
+ * NUL-terminated) and returns the last status code (0 if none). */
+int mv_last_error(char *buf, size_t len);
+
+/* Library ABI version (incremented on any signature change). */
+int mv_abi_version(void);
+
+/* ---------------------------------------------------------------- (a) ---- */
+
+/* Rotate ECI unit vectors into the topocentric ENU frame, all sources.
+ * Replaces the per-time rotation that fills CoordinateRotation.all_coords_topo
+ * (src/matvis/cpu/coords.py:26-38 and the _atioq matmul, :89-116).
+ * eci: (3, ld_eci) f64 rows x,y,z.  rot9: host, row-major 3x3 R_t.
+ * topo: (3, ld_topo) f64 (topo_is_f64=1) or f32.  Rows are evaluated as
+ * (R[i,0]*x + R[i,1]*y) + R[i,2]*z in FP64 without contraction. */
+int mv_rotate(const double *eci, int64_t ld_eci, const double *rot9,
+              int64_t nsrc, void *topo, int topo_is_f64, int64_t ld_topo,
+              mv_stream_t stream);
+
+/* Bytes of workspace mv_rotate_cull needs for a chunk of chunk_len sources. */
+size_t mv_rotate_cull_workspace_bytes(int64_t chunk_len);
+
+/* Fused rotation + horizon cull (z > 0) + stable stream compaction of sources
+ * [chunk0, chunk0 + chunk_len).  Replaces CoordinateRotation.select_chunk
+ * (src/matvis/core/coords.py:99-150): ascending source order is kept, flux is
+ * gathered, and positions n_up..nalloc-1 are zero-filled (coords and flux).
+ * rot9 (host) may be the identity when `eci` already holds ENU vectors.
+ * crd_out: (3, ld_crd), flux_out: (nalloc) [nullable], both f64 if
+ * out_is_f64 else f32.  flux_in: f64 (nsrc) indexed by global source id
+ * [nullable].  idx_out: int32 (nalloc) global source ids [nullable].
+ * n_up_dev: int32 on device, receives the count; if it exceeds nalloc only
+ * nalloc sources are stored and the count still reports the true number
+ * (the caller raises the reference's "nsrc_alloc ... too small" error). */
+int mv_rotate_cull(const double *eci, int64_t ld_eci, const double *rot9,
+                   const double *flux_in, int64_t chunk0, int64_t chunk_len,
+                   int64_t nalloc, void *crd_out, int64_t ld_crd,
+                   void *flux_out, int out_is_f64, int32_t *idx_out,
+                   int32_t *n_up_dev, void *workspace, size_t ws_bytes,
+                   mv_stream_t stream);
+
+/* ---------------------------------------------------------------- (b) ---- */
+
+/* In-place cubic B-spline prefilter of beam planes (mirror boundary), the
+ * prefilter map_coordinates(order=3) applies (src/matvis/gpu/beams.py:187).
+ * planes: (nplanes, nza, naz) complex (is_f64 ? c128 : c64). */
+int mv_beam_prefilter(void *planes, int is_f64, int64_t nplanes, int nza,
+                      int naz, mv_stream_t stream);
+
+/* Interpolate az/za grid beams at topocentric positions.  Replaces
+ * GPUBeamInterpolator._interp + gpu_beam_interpolation
+ * (src/matvis/gpu/beams.py:103-200): (tx, ty) -> (az, za) in the uvbeam
+ * convention, coordinates [za/dza, (az-azmin)/daz], map_coordinates with
+ * mode='constant' (cval 0), order 1 (bilinear) or 3 (bicubic B-spline on
+ * prefiltered planes), then a complex sqrt for power beams.
+ * planes: (nbeam, nax, nfeed, nza, naz) complex, is_f64 selects c128/c64.
+ * grid: device (nbeam, 3) doubles {daz, dza, azmin}.
+ * crd: (3, ld_crd) f64 or f32 (crd_is_f64).  out: (nbeam, nfeed, nax, ld_out)
+ * complex in the planes' precision.  Positions p < n (n = *n_dev if n_dev else
+ * count) are evaluated; positions n..count-1 are set to zero. */
+int mv_beam_interp(const void *planes, int is_f64, int nbeam, int nfeed,
+                   int nax, int nza, int naz, const double *grid, int order,
+                   int is_power, const void *crd, int crd_is_f64,
+                   int64_t ld_crd, int64_t count, const int32_t *n_dev,
+                   void *out, int64_t ld_out, mv_stream_t stream);
+
+/* As mv_beam_interp, but at given (az, za) arrays (f64, count) instead of
+ * topocentric vectors: the reference's gpu_beam_interpolation(beam, daz, dza,
+ * azmin, az, za) utility (src/matvis/gpu/beams.py:127-200). */
+int mv_beam_interp_azza(const void *planes, int is_f64, int nbeam, int nfeed,
+                        int nax, int nza, int naz, const double *grid,
+                        int order, int is_power, const double *az,
+                        const double *za, int64_t count, void *out,
+                        int64_t ld_out, mv_stream_t stream);
+
+/* Analytic Gaussian beam: amplitude exp(-za^2 / (2 sigma_b^2)), divided by
+ * sqrt(2) on every (feed, axis) when polarized.  Replaces the host-side
+ * analytic-beam evaluation of GPUBeamInterpolator.interp
+ * (src/matvis/gpu/beams.py:95-101).  sigma: device (nbeam) radians.
+ * out as for mv_beam_interp, precision out_is_f64. */
+int mv_beam_gaussian(const double *sigma, int nbeam, int polarized,
+                     const void *crd, int crd_is_f64, int64_t ld_crd,
+                     int64_t count, const int32_t *n_dev, void *out,
+                     int out_is_f64, int64_t ld_out, mv_stream_t stream);
+
+/* ---------------------------------------------------------------- (c) ---- */
+
+/* exptau[a, s] = exp(i 2 pi freq (b_a . s) / c).  Replaces TauCalculator
+ * (src/matvis/core/tau.py:48-71).  antpos: device (nant, 3) f64 metres.
+ * crd: (3, ld_crd) f64/f32.  out: (nant, ld_out) complex, out_is_f64.
+ * The phase is formed in FP64 and reduced modulo one turn before sincos. */
+int mv_exptau(const double *antpos, int nant, double freq, const void *crd,
+              int crd_is_f64, int64_t ld_crd, int64_t nsrc, void *out,
+              int out_is_f64, int64_t ld_out, mv_stream_t stream);
+
+/* Z = A * sqrt_flux * exptau in the reference layout (nant*nfeed, nax*nsrc)
+ * with row ant*nfeed+feed and column ax*nsrc+src.  Replaces ZMatrixCalc
+ * (src/matvis/core/getz.py:52-105), including its in-place update
+ * exptau *= sqrt_flux (core/getz.py:80).  beam: (nbeam, nfeed, nax, nsrc);
+ * beam_idx: int32 (nant) [nullable: broadcast beam 0 or per-antenna when
+ * nbeam == nant]. All complex arrays share precision is_f64. */
+int mv_zmat_parts(const void *sqrt_flux, const void *beam,
+                  const int32_t *beam_idx, int nbeam, void *exptau, int nant,
+                  int nfeed, int nax, int64_t nsrc, void *z, int is_f64,
+                  mv_stream_t stream);
+
+/* Fused phase + Z: Z[a*nfeed+f, ax, s] = A[bidx(a), f, ax, s] * flux[s] *
+ * exp(i 2 pi freq (b_a . crd_s) / c), written once, coalesced along s.
+ * The north-star fusion of TauCalculator and ZMatrixCalc
+ * (src/matvis/core/tau.py:65-66, core/getz.py:80-100).
+ * antpos: device (nant, 3) f64 metres.  crd: (3, ld_crd) f64.
+ * flux: f64 sqrt(I/2); read as flux[idx[s]] when idx (int32, count, the
+ * compaction's global source ids) is non-NULL, else as flux[s].
+ * beam: (nbeam, nfeed, nax, ld_beam) complex (beam_is_f64).
+ * form: MV_FORM_*; z0..z2 are the output arrays: z0 the complex matrix for
+ * plain forms; for split forms the hi/mid/lo term arrays (nterms of them) of
+ * interleaved (re, im) 16-bit values.  Layout of every output: row r at
+ * r*ldz (units of complex elements), axis segment ax at ax*seg_stride.
+ * Columns s in [n, ceil(n/pad)*pad) are zero-filled (pad = 32).
+ * scale: multiplies Z before the split (a power of two; 1 for plain). */
+int mv_zmat_fused(const double *antpos, int nant, double freq,
+                  const double *crd, int64_t ld_crd, const double *flux,
+                  const int32_t *idx, const void *beam, int beam_is_f64,
+                  int64_t ld_beam,
+                  int nbeam, const int32_t *beam_idx, int nfeed, int nax,
+                  int64_t count, const int32_t *n_dev, int form, int nterms,
+                  double scale, void *z0, void *z1, void *z2, int64_t ldz,
+                  int64_t seg_stride, mv_stream_t stream);
+
+/* Split a complex matrix (c128 or c64, layout as mv_zmat_fused output) into
+ * nterms 16-bit terms of `form` (SPLIT_BF16 / SPLIT_F16), after multiplying
+ * by `scale`.  Rows [0, rows), columns [0, cols) of each of nseg segments. */
+int mv_split_z(const void *z, int z_is_f64, int64_t rows, int nseg,
+               int64_t cols, int64_t ldz, int64_t seg_stride, int form,
+               int nterms, double scale, void *t0, void *t1, void *t2,
+               int64_t ldt, int64_t tseg_stride, mv_stream_t stream);
+
+/* ---------------------------------------------------------------- (d) ---- */
+
+/* Tile side (rows of V per workgroup tile) used by a HERK form. */
+int mv_herk_tile(int form);
+
+/* Workspace bytes for mv_herk with `nslices` K-slices at row count m. */
+size_t mv_herk_workspace_bytes(int form, int m, int nslices);
+
+/* Hermitian rank-k update over the upper triangle of tiles:
+ * Vacc[r, c] (+)= scale * sum_k conj(Z[r, k]) Z[c, k]   for tile(r) <= tile(c)
+ * Replaces GPUMatMul.compute -> zdotz -> cuBLAS zgemm/cgemm
+ * (src/matvis/gpu/matprod.py:32-36, src/matvis/gpu/_cublas.py:12-65), which
+ * computes the full m x m product.  K is split into `nslices` slices whose
+ * partial tiles land in the workspace and are summed in FP64.
+ * z0..z2: Z (plain) or its split terms (nterms, nprod significant products:
+ * 3 = hi*hi + hi*lo + lo*hi, 4 adds lo*lo, 6 = the 3-term set up to order 2).
+ * Z columns: nseg segments, each seg_len long (*seg_len_dev if non-NULL),
+ * starting at seg*seg_stride; ldz is the row stride; both in complex units.
+ * Vacc: complex128 (ldv x ldv, row-major), ldv >= ntile_side*tile.
+ * accumulate: 0 overwrites the upper tiles, 1 adds to them. */
+int mv_herk(int form, int nterms, int nprod, const void *z0, const void *z1,
+            const void *z2, int m, int64_t ldz, int nseg, int64_t seg_stride,
+            int64_t seg_len, const int32_t *seg_len_dev, int nslices,
+            double scale, double *vacc, int64_t ldv, int accumulate,
+            void *workspace, size_t ws_bytes, mv_stream_t stream);
+
+/* Sum nsrc_v accumulators elementwise into dst (all ldv x ldv c128):
+ * dst = sum_i src_i.  The chunk sum of MatProd.sum_chunks
+ * (src/matvis/core/matprod.py:103-115, src/matvis/gpu/matprod.py:49-51). */
+int mv_vacc_sum(const double *src, int nsrc_v, int64_t stride, double *dst,
+                int64_t count, mv_stream_t stream);
+
+/* Expand the upper-tile accumulator into the API layout (npairs, nf, nf):
+ * out[p, a, b] = V[i*nf + b, j*nf + a] for pair p = (i, j), the feed-swapped
+ * layout of GPUMatMul.sum_chunks (src/matvis/gpu/matprod.py:38-63) and
+ * CPUMatMul (src/matvis/cpu/matprod.py:24-30).  pairs: int32 (npairs, 2)
+ * [nullable: all nant*nant ordered pairs, i-major].  out: c128 or c64. */
+int mv_vis_expand(const double *vacc, int64_t ldv, int tile, int nant,
+                  int nfeed, const int32_t *pairs, int64_t npairs, void *out,
+                  int out_is_f64, mv_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MATVIS_HIP_H */

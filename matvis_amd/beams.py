@@ -1,0 +1,193 @@
+"""Beam models accepted by matvis_amd and their preparation for the device.
+
+The reference takes pyuvdata ``UVBeam`` / ``AnalyticBeam`` / ``BeamInterface``
+objects.  Those are accepted here by duck typing:
+
+* a *grid beam* exposes ``data_array`` shaped
+  ``(Naxes_vec, Nfeeds | Npols, Nfreqs, Naxes2 = n_za, Naxes1 = n_az)``,
+  ``axis1_array`` (azimuth, rad), ``axis2_array`` (zenith angle, rad),
+  ``freq_array``, ``beam_type`` ("efield" or "power") and
+  ``pixel_coordinate_system`` ("az_za") -- the attributes of a pyuvdata UVBeam;
+* an *analytic beam* is :class:`GaussianBeam` (this module);
+* a ``BeamInterface``-like wrapper exposes ``.beam`` (and ``.beam_type``).
+
+:class:`GridBeam` is a small UVBeam stand-in with those attributes.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+C_LIGHT = 299792458.0
+
+
+class GridBeam:
+    """Regular az/za grid beam with the UVBeam attributes matvis reads.
+
+    ``data_array``: (Naxes_vec, Nfeeds, Nfreqs, Nza, Naz) complex for an
+    E-field beam; (1, Npols, Nfreqs, Nza, Naz) real for a power beam.
+    """
+
+    pixel_coordinate_system = "az_za"
+
+    def __init__(self, data_array, az_array, za_array, freq_array=None,
+                 beam_type: str = "efield", feed_array=("x", "y")):
+        self.data_array = np.asarray(data_array)
+        self.axis1_array = np.asarray(az_array, dtype=np.float64)
+        self.axis2_array = np.asarray(za_array, dtype=np.float64)
+        nfreq = self.data_array.shape[2]
+        self.freq_array = (np.zeros(nfreq) if freq_array is None
+                           else np.atleast_1d(np.asarray(freq_array, float)))
+        if beam_type not in ("efield", "power"):
+            raise ValueError("beam_type must be 'efield' or 'power'")
+        self.beam_type = beam_type
+        self.feed_array = list(feed_array)
+        if self.data_array.ndim != 5:
+            raise ValueError("data_array must have 5 axes "
+                             "(Naxes_vec, Nfeeds|Npols, Nfreqs, Nza, Naz)")
+        if self.data_array.shape[3:] != (self.axis2_array.size,
+                                         self.axis1_array.size):
+            raise ValueError("data_array grid does not match az/za arrays")
+
+    @property
+    def Naxes_vec(self):
+        return self.data_array.shape[0]
+
+    @property
+    def Nfeeds(self):
+        return self.data_array.shape[1]
+
+
+@dataclass
+class GaussianBeam:
+    """Circular zenith-pointed Gaussian beam.
+
+    E-field amplitude ``exp(-za^2 / (2 sigma^2))`` (power = its square), with
+    ``sigma`` in radians scaled by ``(freq / reference_frequency) **
+    spectral_index`` when both are set.  ``diameter`` (m) sets sigma from an
+    Airy-disk match, ``arcsin(2.2150894 * lambda / (pi D)) * 2 / 2.355``
+    (unpinned: restates pyuvdata's GaussianBeam, which is absent here).
+    """
+
+    sigma: float | None = None
+    diameter: float | None = None
+    spectral_index: float = 0.0
+    reference_frequency: float | None = None
+    beam_type: str = "efield"
+
+    def __post_init__(self):
+        if (self.sigma is None) == (self.diameter is None):
+            raise ValueError("set exactly one of sigma and diameter")
+
+    def get_sigma(self, freq: float) -> float:
+        if self.diameter is not None:
+            lam = C_LIGHT / freq
+            return float(np.arcsin(2.2150894 * lam / (np.pi * self.diameter))
+                         * 2.0 / 2.355)
+        s = float(self.sigma)
+        if self.reference_frequency and self.spectral_index:
+            s *= (freq / self.reference_frequency) ** self.spectral_index
+        return s
+
+
+class BeamInterface:
+    """Wrapper carrying a beam and the type it is to be used as."""
+
+    def __init__(self, beam, beam_type: str | None = None):
+        self.beam = beam.beam if isinstance(beam, BeamInterface) else beam
+        self.beam_type = beam_type or getattr(self.beam, "beam_type", "efield")
+
+    @property
+    def _isuvbeam(self) -> bool:
+        return is_grid_beam(self.beam)
+
+
+def unwrap(beam):
+    """Return ``(inner beam, beam_type)`` for a beam or a BeamInterface-like."""
+    if hasattr(beam, "beam") and not hasattr(beam, "data_array"):
+        return beam.beam, getattr(beam, "beam_type", None)
+    return beam, getattr(beam, "beam_type", None)
+
+
+def is_grid_beam(beam) -> bool:
+    inner, _ = unwrap(beam)
+    return hasattr(inner, "data_array")
+
+
+def is_analytic_beam(beam) -> bool:
+    inner, _ = unwrap(beam)
+    return hasattr(inner, "get_sigma")
+
+
+def _interp_freq(data: np.ndarray, freqs: np.ndarray, freq: float):
+    """Linear interpolation of data[..., f, :, :] to one frequency."""
+    if data.shape[2] == 1:
+        return data[:, :, 0]
+    order = np.argsort(freqs)
+    freqs = freqs[order]
+    data = data[:, :, order]
+    if not freqs[0] <= freq <= freqs[-1]:
+        raise ValueError(f"frequency {freq} Hz outside the beam's range "
+                         f"[{freqs[0]}, {freqs[-1]}]")
+    k = int(np.clip(np.searchsorted(freqs, freq) - 1, 0, freqs.size - 2))
+    w = (freq - freqs[k]) / (freqs[k + 1] - freqs[k])
+    return (1.0 - w) * data[:, :, k] + w * data[:, :, k + 1]
+
+
+def grid_planes(beam, freq: float, polarized: bool, beam_type=None):
+    """Planes of a grid beam as the device kernels take them.
+
+    Returns ``(planes, daz, dza, azmin, is_power)`` with ``planes`` shaped
+    ``(nax, nfeed, nza, naz')`` -- the ``(nbeam, nax, nfeed, nza, naz)`` layout
+    of gpu/beams.py:143-145 for one beam.
+
+    * frequency: the single plane, or linear interpolation between the two
+      bracketing frequencies (pyuvdata's own frequency interpolation is not
+      restated; unpinned);
+    * unpolarized: an E-field beam becomes the xx power beam
+      ``|E[0, x]|^2 + |E[1, x]|^2`` (core/beams.py:15-32); a power beam keeps
+      its first polarization;
+    * azimuth wrap: a uniform grid covering [az0, az0 + 2 pi) without its end
+      point gets the first column appended at az0 + 2 pi, so sources in the
+      last cell interpolate across the seam;
+    * the map coordinates are [za / dza, (az - azmin) / daz]
+      (gpu/beams.py:185), so the za grid must start at 0.
+    """
+    inner, bt = unwrap(beam)
+    bt = beam_type or bt or getattr(inner, "beam_type", "efield")
+    if getattr(inner, "pixel_coordinate_system", "az_za") != "az_za":
+        raise ValueError('pixel coordinate system must be "az_za"')
+    az = np.asarray(inner.axis1_array, dtype=np.float64)
+    za = np.asarray(inner.axis2_array, dtype=np.float64)
+    if az.size < 2 or za.size < 2:
+        raise ValueError("a grid beam needs at least 2 points on each axis")
+    if abs(za[0]) > 1e-12:
+        raise ValueError("the zenith-angle grid must start at 0")
+    data = _interp_freq(np.asarray(inner.data_array),
+                        np.asarray(inner.freq_array, float), freq)
+    # data: (Naxes_vec, Nfeeds|Npols, nza, naz); its meaning follows the
+    # stored beam's own type, the requested use follows `bt`
+    stored = getattr(inner, "beam_type", "efield")
+    if polarized:
+        if bt != "efield" or stored != "efield":
+            raise ValueError("beam type must be efield if using polarized=True")
+        planes = data[:2, :2].astype(np.complex128)
+        is_power = False
+    else:
+        if stored == "efield":
+            e = data[:, 0]  # feed x, every vector axis
+            power = (np.abs(e) ** 2).sum(axis=0)
+        else:
+            power = np.real(data[0, 0])
+        planes = power[None, None].astype(np.float64)
+        is_power = True
+    daz = float(az[1] - az[0])
+    dza = float(za[1] - za[0])
+    azmin = float(az.min())
+    uniform = np.allclose(np.diff(az), daz, rtol=1e-9, atol=1e-12)
+    span = az[-1] - az[0]
+    if uniform and abs(span + daz - 2 * np.pi) < 1e-6 * daz:
+        planes = np.concatenate([planes, planes[..., :1]], axis=-1)
+    return planes, daz, dza, azmin, is_power

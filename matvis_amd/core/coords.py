@@ -1,0 +1,171 @@
+"""Coordinate rotation plugin: ECI -> topocentric ENU, horizon cull, compaction.
+
+Contract of matvis's ``CoordinateRotation`` (core/coords.py:20-179): a
+registry of subclasses by class name, ``setup()`` allocating the buffers,
+``rotate(t)`` for one time index, and ``select_chunk(chunk, t)`` returning the
+above-horizon coordinates, their fluxes (zero past ``n``) and ``n``.
+
+Subclasses provide the per-time rotation either as a 3x3 matrix R_t
+(:meth:`rotation_matrix`; the device applies it fused with the cull) or by
+filling topocentric vectors themselves (:meth:`set_topocentric`).
+"""
+
+from __future__ import annotations
+
+from abc import ABC
+
+import numpy as np
+
+from .. import _device as D
+from .._lib import call, lib
+from .._utils import get_dtypes
+from ..astro import skycoord_ra_dec
+from ..coordinates import point_source_crd_eq
+
+_EYE = np.eye(3, dtype=np.float64)
+
+
+class CoordinateRotation(ABC):
+    """Base class; every subclass registers itself in ``_methods``."""
+
+    _methods: dict = {}
+    requires_gpu: bool = False
+
+    def __init_subclass__(cls) -> None:
+        CoordinateRotation._methods[cls.__name__] = cls
+        return super().__init_subclass__()
+
+    def __init__(
+        self,
+        flux: np.ndarray,
+        times,
+        telescope_loc,
+        skycoords,
+        chunk_size: int | None = None,
+        source_buffer: float = 0.55,
+        precision: int = 1,
+        gpu: bool = True,
+    ):
+        if not gpu:
+            raise ValueError(
+                "matvis_amd coordinate rotations run on the GPU (gpu=True)"
+            )
+        self.gpu = True
+        self.precision = precision
+        self.rtype, self.ctype = get_dtypes(precision)
+        flux = np.asarray(flux)
+        if np.iscomplexobj(flux) or flux.ndim != 1:
+            raise NotImplementedError(
+                "polarized (coherency-matrix) sky models are not supported; "
+                "I_sky must be real with shape (NSRCS,)"
+            )
+        self.sky_model_dtype = self.rtype
+        self.flux = flux.astype(np.float64)
+        self._polarized = False
+        self.nsrc = len(flux)
+        self.times = times
+        self.telescope_loc = telescope_loc
+        self.skycoords = skycoords
+        if hasattr(times, "ndim"):
+            assert times.ndim == 1
+        assert len(skycoords) == self.nsrc
+        self.chunk_size = chunk_size or self.nsrc
+        self.source_buffer = source_buffer
+        # allocation rule of core/coords.py:79-82
+        if self.chunk_size > 1000:
+            self.nsrc_alloc = int(self.chunk_size * self.source_buffer)
+        else:
+            self.nsrc_alloc = self.chunk_size
+        self._rot = None
+        self._mode = None  # "eci" (R_t on device) or "topo" (filled vectors)
+        self._topo_valid = False
+
+    # ----------------------------------------------------------- buffers
+
+    def setup(self):
+        """Allocate device buffers and upload source vectors and fluxes."""
+        ra, dec = skycoord_ra_dec(self.skycoords)
+        self.eci = D.to_dev(point_source_crd_eq(ra, dec), np.float64)
+        self.flux_dev = D.to_dev(self.flux, np.float64)
+        self._topo = D.zeros((3, self.nsrc), self.rtype)
+        self._topo64 = None
+        self.coords_above_horizon = D.zeros((3, self.nsrc_alloc), self.rtype)
+        self.flux_above_horizon = D.zeros((self.nsrc_alloc,), self.rtype)
+        self.n_up_dev = D.zeros((1,), np.int32)
+        nbytes = lib().mv_rotate_cull_workspace_bytes(self.chunk_size)
+        self.workspace = D.empty((nbytes,), np.uint8)
+
+    # ----------------------------------------------------------- rotation
+
+    def rotation_matrix(self, t: int) -> np.ndarray | None:
+        """R_t (3x3, ECI -> ENU) for time index t, or None if the subclass
+        fills topocentric vectors itself."""
+        return None
+
+    def rotate(self, t: int) -> None:
+        """Prepare the rotation for time index t."""
+        rot = self.rotation_matrix(t)
+        if rot is None:
+            raise NotImplementedError(
+                f"{type(self).__name__} must implement rotation_matrix() or "
+                "override rotate() and call set_topocentric()"
+            )
+        self._rot = np.ascontiguousarray(rot, dtype=np.float64)
+        self._mode = "eci"
+        self._topo_valid = False
+
+    def set_topocentric(self, enu: np.ndarray) -> None:
+        """Use host-computed ENU unit vectors (3, nsrc) for the current time."""
+        if self._topo64 is None:
+            self._topo64 = D.zeros((3, self.nsrc), np.float64)
+        self._topo64.copy_(D.to_dev(np.asarray(enu, dtype=np.float64)))
+        self._rot = _EYE
+        self._mode = "topo"
+        self._topo_valid = False
+
+    @property
+    def all_coords_topo(self):
+        """Topocentric unit vectors of every source, (3, nsrc), rtype."""
+        if self._mode is None:
+            raise RuntimeError("call rotate(t) first")
+        if not self._topo_valid:
+            src = self.eci if self._mode == "eci" else self._topo64
+            call("mv_rotate", D.ptr(src), self.nsrc, self._rot.ctypes.data,
+                 self.nsrc, D.ptr(self._topo), int(self.precision == 2),
+                 self.nsrc, D.stream_ptr())
+            self._topo_valid = True
+        return self._topo
+
+    # ----------------------------------------------------------- cull
+
+    def select_chunk_async(self, chunk: int, t: int):
+        """Enqueue the fused rotate/cull/compaction of one chunk; the count
+        stays on the device in ``n_up_dev``."""
+        if self._mode is None:
+            raise RuntimeError("call rotate(t) first")
+        src = self.eci if self._mode == "eci" else self._topo64
+        chunk0 = chunk * self.chunk_size
+        chunk_len = max(0, min(self.chunk_size, self.nsrc - chunk0))
+        chunk0 = min(chunk0, self.nsrc)
+        call(
+            "mv_rotate_cull", D.ptr(src), self.nsrc, self._rot.ctypes.data,
+            D.ptr(self.flux_dev), chunk0, chunk_len, self.nsrc_alloc,
+            D.ptr(self.coords_above_horizon), self.nsrc_alloc,
+            D.ptr(self.flux_above_horizon), int(self.precision == 2), None,
+            D.ptr(self.n_up_dev), D.ptr(self.workspace),
+            self.workspace.numel(), D.stream_ptr(),
+        )
+        return self.coords_above_horizon, self.flux_above_horizon, self.n_up_dev
+
+    def select_chunk(self, chunk: int, t: int):
+        """Cull and compact one chunk; returns ``(crd, flux, n)`` with the
+        reference's error if ``n`` exceeds ``nsrc_alloc``
+        (core/coords.py:117-121)."""
+        crd, flux, n_dev = self.select_chunk_async(chunk, t)
+        n = int(n_dev.item())
+        if n > self.nsrc_alloc:
+            raise ValueError(
+                f"nsrc_alloc ({self.nsrc_alloc}) is too small for the number of "
+                f"sources above horizon ({n}). Try increasing source_buffer."
+            )
+        return crd, flux, n

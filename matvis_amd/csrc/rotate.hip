@@ -1,0 +1,245 @@
+// (a) Rotation of ECI source vectors into the topocentric ENU frame, fused
+// with the horizon cull and a stable (ascending-index) stream compaction.
+//
+// Compaction runs as count -> scan -> scatter.  Each block owns a fixed run of
+// kItemsPerBlock sources and re-derives the same horizon flags in both the
+// count and the scatter pass, so the output order equals the reference's
+// `xp.where(topo[2] > 0)` order (src/matvis/core/coords.py:115) for any grid.
+#include "common.h"
+
+namespace mv {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kRounds = 8;
+constexpr int kItemsPerBlock = kThreads * kRounds;  // 2048 sources per block
+
+struct Mat3 {
+  double m[9];
+};
+
+// Row i of R . (x, y, z), evaluated as (R[i,0]*x + R[i,1]*y) + R[i,2]*z with
+// one rounding per operation, so the host oracle reproduces it bit for bit.
+__device__ __forceinline__ double rot_row(const Mat3 &R, int i, double x,
+                                          double y, double z) {
+#pragma clang fp contract(off)
+  double a = R.m[3 * i + 0] * x;
+  double b = R.m[3 * i + 1] * y;
+  double c = R.m[3 * i + 2] * z;
+  return (a + b) + c;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void rotate_kernel(
+    const double *__restrict__ eci, int64_t ld_eci, Mat3 R, int64_t nsrc,
+    T *__restrict__ topo, int64_t ld_topo) {
+  int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= nsrc) return;
+  double x = eci[p], y = eci[ld_eci + p], z = eci[2 * ld_eci + p];
+  topo[p] = (T)rot_row(R, 0, x, y, z);
+  topo[ld_topo + p] = (T)rot_row(R, 1, x, y, z);
+  topo[2 * ld_topo + p] = (T)rot_row(R, 2, x, y, z);
+}
+
+__device__ __forceinline__ bool above_horizon(const double *__restrict__ eci,
+                                              int64_t ld_eci, const Mat3 &R,
+                                              int64_t g) {
+  double x = eci[g], y = eci[ld_eci + g], z = eci[2 * ld_eci + g];
+  return rot_row(R, 2, x, y, z) > 0.0;
+}
+
+// Pass 1: number of above-horizon sources in each block's run.
+__global__ __launch_bounds__(kThreads) void cull_count_kernel(
+    const double *__restrict__ eci, int64_t ld_eci, Mat3 R, int64_t chunk0,
+    int64_t chunk_len, int32_t *__restrict__ counts) {
+  __shared__ int wave_sum[kThreads / 64];
+  const int64_t base = (int64_t)blockIdx.x * kItemsPerBlock;
+  int local = 0;
+#pragma unroll
+  for (int r = 0; r < kRounds; ++r) {
+    int64_t p = base + r * kThreads + threadIdx.x;
+    if (p < chunk_len && above_horizon(eci, ld_eci, R, chunk0 + p)) ++local;
+  }
+  // wave reduction
+  for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
+  if ((threadIdx.x & 63) == 0) wave_sum[threadIdx.x >> 6] = local;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < kThreads / 64; ++w) s += wave_sum[w];
+    counts[blockIdx.x] = s;
+  }
+}
+
+// Pass 2: exclusive scan of block counts (single workgroup); total -> n_up.
+__global__ __launch_bounds__(1024) void cull_scan_kernel(
+    int32_t *__restrict__ counts, int64_t nblocks,
+    int32_t *__restrict__ n_up) {
+  __shared__ int buf[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t b0 = 0; b0 < nblocks; b0 += 1024) {
+    int64_t b = b0 + threadIdx.x;
+    int v = b < nblocks ? counts[b] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 1024 entries
+    for (int off = 1; off < 1024; off <<= 1) {
+      int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    int incl = buf[threadIdx.x];
+    if (b < nblocks) counts[b] = carry + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *n_up = carry;
+}
+
+// Pass 3: recompute flags, place each kept source at its block offset plus its
+// rank inside the block, then zero-fill positions [n_up, nalloc).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void cull_scatter_kernel(
+    const double *__restrict__ eci, int64_t ld_eci, Mat3 R,
+    const double *__restrict__ flux_in, int64_t chunk0, int64_t chunk_len,
+    int64_t nalloc, const int32_t *__restrict__ offsets,
+    const int32_t *__restrict__ n_up_dev, T *__restrict__ crd, int64_t ld_crd,
+    T *__restrict__ flux_out, int32_t *__restrict__ idx_out,
+    int64_t nblocks_real) {
+  __shared__ int wave_cnt[kThreads / 64];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if ((int64_t)blockIdx.x < nblocks_real) {
+    const int64_t base = (int64_t)blockIdx.x * kItemsPerBlock;
+    int64_t run = offsets[blockIdx.x];
+    for (int r = 0; r < kRounds; ++r) {
+      int64_t p = base + r * kThreads + threadIdx.x;
+      int64_t g = chunk0 + p;
+      bool keep = false;
+      double x = 0, y = 0, z = 0;
+      if (p < chunk_len) {
+        x = eci[g];
+        y = eci[ld_eci + g];
+        z = eci[2 * ld_eci + g];
+        keep = rot_row(R, 2, x, y, z) > 0.0;
+      }
+      uint64_t mask = __ballot(keep);
+      int rank = __popcll(mask & ((1ull << lane) - 1ull));
+      if (lane == 0) wave_cnt[wave] = __popcll(mask);
+      __syncthreads();
+      int before = 0, total = 0;
+#pragma unroll
+      for (int w = 0; w < kThreads / 64; ++w) {
+        before += (w < wave) ? wave_cnt[w] : 0;
+        total += wave_cnt[w];
+      }
+      if (keep) {
+        int64_t dst = run + before + rank;
+        if (dst < nalloc) {
+          crd[dst] = (T)rot_row(R, 0, x, y, z);
+          crd[ld_crd + dst] = (T)rot_row(R, 1, x, y, z);
+          crd[2 * ld_crd + dst] = (T)rot_row(R, 2, x, y, z);
+          if (flux_out) flux_out[dst] = (T)flux_in[g];
+          if (idx_out) idx_out[dst] = (int32_t)g;
+        }
+      }
+      run += total;
+      __syncthreads();
+    }
+  }
+  // zero-fill the tail [n_up, nalloc)
+  const int64_t n_up = *n_up_dev;
+  const int64_t stride = (int64_t)gridDim.x * kThreads;
+  for (int64_t q = n_up + (int64_t)blockIdx.x * kThreads + threadIdx.x;
+       q < nalloc; q += stride) {
+    crd[q] = (T)0;
+    crd[ld_crd + q] = (T)0;
+    crd[2 * ld_crd + q] = (T)0;
+    if (flux_out) flux_out[q] = (T)0;
+    if (idx_out) idx_out[q] = -1;
+  }
+}
+
+Mat3 load_mat(const double *rot9) {
+  Mat3 R;
+  for (int i = 0; i < 9; ++i) R.m[i] = rot9[i];
+  return R;
+}
+
+}  // namespace
+}  // namespace mv
+
+using namespace mv;
+
+extern "C" int mv_rotate(const double *eci, int64_t ld_eci, const double *rot9,
+                         int64_t nsrc, void *topo, int topo_is_f64,
+                         int64_t ld_topo, mv_stream_t stream) {
+  MV_REQUIRE(eci && rot9 && topo, "mv_rotate: null pointer");
+  MV_REQUIRE(nsrc >= 0 && ld_eci >= nsrc && ld_topo >= nsrc,
+             "mv_rotate: bad sizes nsrc=%lld ld_eci=%lld ld_topo=%lld",
+             (long long)nsrc, (long long)ld_eci, (long long)ld_topo);
+  if (nsrc == 0) return MV_OK;
+  Mat3 R = load_mat(rot9);
+  dim3 grid((unsigned)cdiv(nsrc, kThreads));
+  if (topo_is_f64)
+    rotate_kernel<double><<<grid, kThreads, 0, as_stream(stream)>>>(
+        eci, ld_eci, R, nsrc, (double *)topo, ld_topo);
+  else
+    rotate_kernel<float><<<grid, kThreads, 0, as_stream(stream)>>>(
+        eci, ld_eci, R, nsrc, (float *)topo, ld_topo);
+  return check_launch("mv_rotate");
+}
+
+extern "C" size_t mv_rotate_cull_workspace_bytes(int64_t chunk_len) {
+  int64_t nb = chunk_len > 0 ? cdiv(chunk_len, kItemsPerBlock) : 1;
+  return (size_t)(cdiv(nb * 4, 256) * 256);
+}
+
+extern "C" int mv_rotate_cull(const double *eci, int64_t ld_eci,
+                              const double *rot9, const double *flux_in,
+                              int64_t chunk0, int64_t chunk_len,
+                              int64_t nalloc, void *crd_out, int64_t ld_crd,
+                              void *flux_out, int out_is_f64, int32_t *idx_out,
+                              int32_t *n_up_dev, void *workspace,
+                              size_t ws_bytes, mv_stream_t stream) {
+  MV_REQUIRE(eci && rot9 && crd_out && n_up_dev && workspace,
+             "mv_rotate_cull: null pointer");
+  MV_REQUIRE(!flux_out || flux_in,
+             "mv_rotate_cull: flux_out requires flux_in");
+  MV_REQUIRE(chunk0 >= 0 && chunk_len >= 0 && nalloc >= 0 &&
+                 ld_crd >= nalloc && chunk0 + chunk_len <= ld_eci,
+             "mv_rotate_cull: bad sizes chunk0=%lld chunk_len=%lld "
+             "nalloc=%lld ld_crd=%lld ld_eci=%lld",
+             (long long)chunk0, (long long)chunk_len, (long long)nalloc,
+             (long long)ld_crd, (long long)ld_eci);
+  MV_REQUIRE(chunk0 + chunk_len < (int64_t)INT32_MAX,
+             "mv_rotate_cull: source index exceeds int32");
+  MV_REQUIRE(ws_bytes >= mv_rotate_cull_workspace_bytes(chunk_len),
+             "mv_rotate_cull: workspace too small (%zu < %zu)", ws_bytes,
+             mv_rotate_cull_workspace_bytes(chunk_len));
+  hipStream_t s = as_stream(stream);
+  Mat3 R = load_mat(rot9);
+  int32_t *counts = (int32_t *)workspace;
+  int64_t nb = cdiv(chunk_len, kItemsPerBlock);
+  if (nb > 0) {
+    cull_count_kernel<<<(unsigned)nb, kThreads, 0, s>>>(eci, ld_eci, R, chunk0,
+                                                        chunk_len, counts);
+    if (int e = check_launch("mv_rotate_cull/count")) return e;
+  }
+  cull_scan_kernel<<<1, 1024, 0, s>>>(counts, nb, n_up_dev);
+  if (int e = check_launch("mv_rotate_cull/scan")) return e;
+  int64_t grid = nb > 0 ? nb : 1;
+  if (out_is_f64)
+    cull_scatter_kernel<double><<<(unsigned)grid, kThreads, 0, s>>>(
+        eci, ld_eci, R, flux_in, chunk0, chunk_len, nalloc, counts, n_up_dev,
+        (double *)crd_out, ld_crd, (double *)flux_out, idx_out, nb);
+  else
+    cull_scatter_kernel<float><<<(unsigned)grid, kThreads, 0, s>>>(
+        eci, ld_eci, R, flux_in, chunk0, chunk_len, nalloc, counts, n_up_dev,
+        (float *)crd_out, ld_crd, (float *)flux_out, idx_out, nb);
+  return check_launch("mv_rotate_cull/scatter");
+}

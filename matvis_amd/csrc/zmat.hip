@@ -1,0 +1,396 @@
+// (c) Geometric phase and the Z matrix.
+//
+// Phase: cycles = (b . s) * (freq / c) in FP64, reduced to [-1/2, 1/2] turns
+// before sincos, so the sincos argument is small whatever the baseline length
+// (the reference forms exp(i * 2 pi freq (b . s) / c) directly,
+// src/matvis/core/tau.py:30-31,65-66).  FP64 forms take an FP64 sincospi of
+// the reduced phase; FP32 and split forms take the FP32 sincospi.
+#include "common.h"
+
+namespace mv {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kZPad = 32;  // Z columns are zero-padded to a multiple of this
+
+__device__ __forceinline__ double reduced_turns(const double *__restrict__ ap,
+                                                double fc, double sx,
+                                                double sy, double sz) {
+  double cyc = (ap[0] * sx + ap[1] * sy + ap[2] * sz) * fc;
+  return cyc - rint(cyc);
+}
+
+template <typename TO> struct Phasor;
+template <> struct Phasor<double> {
+  __device__ static cplx<double> eval(double turns) {
+    double s, c;
+    sincospi(2.0 * turns, &s, &c);
+    return {c, s};
+  }
+};
+template <> struct Phasor<float> {
+  __device__ static cplx<float> eval(double turns) {
+    float s, c;
+    sincospif((float)(2.0 * turns), &s, &c);
+    return {c, s};
+  }
+};
+
+template <typename TC, typename TO>
+__global__ __launch_bounds__(kThreads) void exptau_kernel(
+    const double *__restrict__ antpos, double fc, const TC *__restrict__ crd,
+    int64_t ld_crd, int64_t nsrc, cplx<TO> *__restrict__ out,
+    int64_t ld_out) {
+  int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  int a = blockIdx.y;
+  if (s >= nsrc) return;
+  double t = reduced_turns(antpos + 3 * a, fc, (double)crd[s],
+                           (double)crd[ld_crd + s], (double)crd[2 * ld_crd + s]);
+  out[(int64_t)a * ld_out + s] = Phasor<TO>::eval(t);
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void zmat_parts_kernel(
+    const T *__restrict__ sqrt_flux, const cplx<T> *__restrict__ beam,
+    const int32_t *__restrict__ beam_idx, int nbeam, cplx<T> *exptau, int nf,
+    int na, int64_t nsrc, cplx<T> *__restrict__ z) {
+  int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  int a = blockIdx.y;
+  if (s >= nsrc) return;
+  cplx<T> e = exptau[(int64_t)a * nsrc + s];
+  T f = sqrt_flux[s];
+  e.re *= f;
+  e.im *= f;
+  exptau[(int64_t)a * nsrc + s] = e;  // core/getz.py:80 updates exptau in place
+  int b = beam_idx ? beam_idx[a] : (nbeam == 1 ? 0 : a);
+  for (int fd = 0; fd < nf; ++fd)
+    for (int ax = 0; ax < na; ++ax) {
+      cplx<T> A = beam[((int64_t)(b * nf + fd) * na + ax) * nsrc + s];
+      z[((int64_t)a * nf + fd) * (na * nsrc) + (int64_t)ax * nsrc + s] =
+          cmul(e, A);
+    }
+}
+
+// ---------------------------------------------------------------- splitting
+
+__device__ __forceinline__ uint32_t bf16_bits_rn(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+__device__ __forceinline__ float bf16_value(uint32_t h) {
+  return __uint_as_float(h << 16);
+}
+__device__ __forceinline__ uint32_t f16_bits_rn(float x) {
+  _Float16 h = (_Float16)x;
+  return (uint32_t)__builtin_bit_cast(uint16_t, h);
+}
+__device__ __forceinline__ float f16_value(uint32_t h) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
+}
+
+// Splits x into NT 16-bit terms x ~ t0 + t1 + ..., each the round-to-nearest
+// of the remainder the previous terms left (exact in FP32).
+template <bool F16, int NT>
+__device__ __forceinline__ void split_terms(float x, uint32_t *t) {
+  float r = x;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    uint32_t b = F16 ? f16_bits_rn(r) : bf16_bits_rn(r);
+    t[k] = b;
+    r = r - (F16 ? f16_value(b) : bf16_value(b));
+  }
+}
+
+template <int FORM, int NT> struct ZWriter;
+template <> struct ZWriter<MV_FORM_C128, 1> {
+  typedef double T;
+  __device__ static void put(void *const *zs, int64_t off, cplx<double> v) {
+    ((cplx<double> *)zs[0])[off] = v;
+  }
+};
+template <> struct ZWriter<MV_FORM_C64, 1> {
+  typedef float T;
+  __device__ static void put(void *const *zs, int64_t off, cplx<float> v) {
+    ((cplx<float> *)zs[0])[off] = v;
+  }
+};
+template <int NT> struct ZWriter<MV_FORM_SPLIT_BF16, NT> {
+  typedef float T;
+  __device__ static void put(void *const *zs, int64_t off, cplx<float> v) {
+    uint32_t r[NT], i[NT];
+    split_terms<false, NT>(v.re, r);
+    split_terms<false, NT>(v.im, i);
+#pragma unroll
+    for (int k = 0; k < NT; ++k) ((uint32_t *)zs[k])[off] = r[k] | (i[k] << 16);
+  }
+};
+template <int NT> struct ZWriter<MV_FORM_SPLIT_F16, NT> {
+  typedef float T;
+  __device__ static void put(void *const *zs, int64_t off, cplx<float> v) {
+    uint32_t r[NT], i[NT];
+    split_terms<true, NT>(v.re, r);
+    split_terms<true, NT>(v.im, i);
+#pragma unroll
+    for (int k = 0; k < NT; ++k) ((uint32_t *)zs[k])[off] = r[k] | (i[k] << 16);
+  }
+};
+
+struct ZOut {
+  void *z[3];
+};
+
+// One thread per source, ants_per_block antennas per block (grid.y).
+// POL: nfeed = nax = 2 (polarized) or 1.
+template <int FORM, int NT, bool POL>
+__global__ __launch_bounds__(kThreads) void zmat_fused_kernel(
+    const double *__restrict__ antpos, int nant, double fc,
+    const double *__restrict__ crd, int64_t ld_crd,
+    const double *__restrict__ flux, const int32_t *__restrict__ idx,
+    const cplx<typename ZWriter<FORM, NT>::T> *__restrict__ beam,
+    int64_t ld_beam, int nbeam, const int32_t *__restrict__ beam_idx,
+    int64_t count, const int32_t *__restrict__ n_dev, double scale,
+    ZOut zo, int64_t ldz, int64_t seg_stride, int ants_per_block) {
+  typedef typename ZWriter<FORM, NT>::T T;
+  constexpr int nf = POL ? 2 : 1;
+  constexpr int na = POL ? 2 : 1;
+  const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  const int64_t n = n_dev ? min((int64_t)*n_dev, count) : count;
+  const int64_t n_pad = cdiv(n, kZPad) * kZPad;
+  if (s >= n_pad || s >= seg_stride) return;
+  const int a0 = blockIdx.y * ants_per_block;
+  const int a1 = min(a0 + ants_per_block, nant);
+  if (s >= n) {
+    const cplx<T> zero = {(T)0, (T)0};
+    for (int a = a0; a < a1; ++a)
+      for (int fd = 0; fd < nf; ++fd)
+        for (int ax = 0; ax < na; ++ax)
+          ZWriter<FORM, NT>::put(zo.z, ((int64_t)a * nf + fd) * ldz +
+                                           (int64_t)ax * seg_stride + s,
+                                 zero);
+    return;
+  }
+  const double sx = crd[s], sy = crd[ld_crd + s], sz = crd[2 * ld_crd + s];
+  const T amp = (T)(flux[idx ? (int64_t)idx[s] : s] * scale);
+  int cached = -1;
+  cplx<T> A[nf * na];
+  for (int a = a0; a < a1; ++a) {
+    const int b = beam_idx ? beam_idx[a] : (nbeam == 1 ? 0 : a);
+    if (b != cached) {
+#pragma unroll
+      for (int q = 0; q < nf * na; ++q)
+        A[q] = beam[((int64_t)b * nf * na + q) * ld_beam + s];
+      cached = b;
+    }
+    cplx<T> e = Phasor<T>::eval(reduced_turns(antpos + 3 * a, fc, sx, sy, sz));
+    e.re *= amp;
+    e.im *= amp;
+#pragma unroll
+    for (int fd = 0; fd < nf; ++fd)
+#pragma unroll
+      for (int ax = 0; ax < na; ++ax)
+        ZWriter<FORM, NT>::put(
+            zo.z, ((int64_t)a * nf + fd) * ldz + (int64_t)ax * seg_stride + s,
+            cmul(A[fd * na + ax], e));
+  }
+}
+
+template <typename TI, int FORM, int NT>
+__global__ __launch_bounds__(kThreads) void split_kernel(
+    const cplx<TI> *__restrict__ z, int64_t rows, int nseg, int64_t cols,
+    int64_t ldz, int64_t seg_stride, float scale, ZOut to, int64_t ldt,
+    int64_t tseg_stride) {
+  int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  int64_t rs = blockIdx.y;  // row * nseg + seg
+  if (c >= cols) return;
+  int64_t r = rs / nseg, sg = rs % nseg;
+  cplx<TI> v = z[r * ldz + sg * seg_stride + c];
+  cplx<float> w = {(float)v.re * scale, (float)v.im * scale};
+  ZWriter<FORM, NT>::put(to.z, r * ldt + sg * tseg_stride + c, w);
+}
+
+template <int FORM, int NT>
+int launch_fused(dim3 g, hipStream_t st, const double *antpos, int nant,
+                 double fc, const double *crd, int64_t ld_crd,
+                 const double *flux, const int32_t *idx, const void *beam,
+                 int64_t ld_beam,
+                 int nbeam, const int32_t *beam_idx, int nf, int na,
+                 int64_t count, const int32_t *n_dev, double scale, ZOut zo,
+                 int64_t ldz, int64_t seg_stride, int apb) {
+  typedef typename ZWriter<FORM, NT>::T T;
+  (void)na;
+  if (nf == 2)
+    zmat_fused_kernel<FORM, NT, true><<<g, kThreads, 0, st>>>(
+        antpos, nant, fc, crd, ld_crd, flux, idx, (const cplx<T> *)beam,
+        ld_beam, nbeam, beam_idx, count, n_dev, scale, zo, ldz, seg_stride,
+        apb);
+  else
+    zmat_fused_kernel<FORM, NT, false><<<g, kThreads, 0, st>>>(
+        antpos, nant, fc, crd, ld_crd, flux, idx, (const cplx<T> *)beam,
+        ld_beam, nbeam, beam_idx, count, n_dev, scale, zo, ldz, seg_stride,
+        apb);
+  return check_launch("mv_zmat_fused");
+}
+
+}  // namespace
+}  // namespace mv
+
+using namespace mv;
+
+extern "C" int mv_exptau(const double *antpos, int nant, double freq,
+                         const void *crd, int crd_is_f64, int64_t ld_crd,
+                         int64_t nsrc, void *out, int out_is_f64,
+                         int64_t ld_out, mv_stream_t stream) {
+  MV_REQUIRE(antpos && crd && out, "mv_exptau: null pointer");
+  MV_REQUIRE(nant >= 1 && nant <= 65535 && nsrc >= 0 && ld_crd >= nsrc &&
+                 ld_out >= nsrc,
+             "mv_exptau: bad sizes");
+  if (nsrc == 0) return MV_OK;
+  dim3 g((unsigned)cdiv(nsrc, kThreads), (unsigned)nant);
+  hipStream_t s = as_stream(stream);
+  const double fc = freq / kC;
+#define MV_TAU(TC, TO)                                                       \
+  exptau_kernel<TC, TO><<<g, kThreads, 0, s>>>(                              \
+      antpos, fc, (const TC *)crd, ld_crd, nsrc, (cplx<TO> *)out, ld_out)
+  if (crd_is_f64) {
+    if (out_is_f64) MV_TAU(double, double);
+    else MV_TAU(double, float);
+  } else {
+    if (out_is_f64) MV_TAU(float, double);
+    else MV_TAU(float, float);
+  }
+#undef MV_TAU
+  return check_launch("mv_exptau");
+}
+
+extern "C" int mv_zmat_parts(const void *sqrt_flux, const void *beam,
+                             const int32_t *beam_idx, int nbeam, void *exptau,
+                             int nant, int nfeed, int nax, int64_t nsrc,
+                             void *z, int is_f64, mv_stream_t stream) {
+  MV_REQUIRE(sqrt_flux && beam && exptau && z, "mv_zmat_parts: null pointer");
+  MV_REQUIRE(nant >= 1 && nant <= 65535 && nbeam >= 1 && nsrc >= 0 &&
+                 (nfeed == 1 || nfeed == 2) && (nax == 1 || nax == 2),
+             "mv_zmat_parts: bad sizes");
+  MV_REQUIRE(beam_idx || nbeam == 1 || nbeam == nant,
+             "mv_zmat_parts: nbeam must be 1 or nant without beam_idx");
+  if (nsrc == 0) return MV_OK;
+  dim3 g((unsigned)cdiv(nsrc, kThreads), (unsigned)nant);
+  hipStream_t s = as_stream(stream);
+  if (is_f64)
+    zmat_parts_kernel<double><<<g, kThreads, 0, s>>>(
+        (const double *)sqrt_flux, (const cplx<double> *)beam, beam_idx,
+        nbeam, (cplx<double> *)exptau, nfeed, nax, nsrc, (cplx<double> *)z);
+  else
+    zmat_parts_kernel<float><<<g, kThreads, 0, s>>>(
+        (const float *)sqrt_flux, (const cplx<float> *)beam, beam_idx, nbeam,
+        (cplx<float> *)exptau, nfeed, nax, nsrc, (cplx<float> *)z);
+  return check_launch("mv_zmat_parts");
+}
+
+extern "C" int mv_zmat_fused(const double *antpos, int nant, double freq,
+                             const double *crd, int64_t ld_crd,
+                             const double *flux, const int32_t *idx,
+                             const void *beam, int beam_is_f64,
+                             int64_t ld_beam, int nbeam,
+                             const int32_t *beam_idx, int nfeed, int nax,
+                             int64_t count, const int32_t *n_dev, int form,
+                             int nterms, double scale, void *z0, void *z1,
+                             void *z2, int64_t ldz, int64_t seg_stride,
+                             mv_stream_t stream) {
+  MV_REQUIRE(antpos && crd && flux && beam && z0,
+             "mv_zmat_fused: null pointer");
+  MV_REQUIRE(nant >= 1 && nbeam >= 1 && count >= 0 && ld_crd >= count &&
+                 ld_beam >= count && (nfeed == 1 || nfeed == 2) &&
+                 nax == nfeed,
+             "mv_zmat_fused: bad sizes (nfeed and nax must both be 1 or 2)");
+  MV_REQUIRE(beam_idx || nbeam == 1 || nbeam == nant,
+             "mv_zmat_fused: nbeam must be 1 or nant without beam_idx");
+  MV_REQUIRE(seg_stride >= cdiv(count, kZPad) * kZPad,
+             "mv_zmat_fused: seg_stride (%lld) must cover count rounded up "
+             "to %d",
+             (long long)seg_stride, kZPad);
+  MV_REQUIRE(ldz >= (int64_t)nax * seg_stride,
+             "mv_zmat_fused: ldz must be >= nax * seg_stride");
+  const bool plain = form == MV_FORM_C128 || form == MV_FORM_C64;
+  MV_REQUIRE(plain || form == MV_FORM_SPLIT_BF16 || form == MV_FORM_SPLIT_F16,
+             "mv_zmat_fused: unknown form %d", form);
+  MV_REQUIRE(plain ? nterms == 1 : (nterms == 2 || nterms == 3),
+             "mv_zmat_fused: bad nterms %d for form %d", nterms, form);
+  MV_REQUIRE(!!beam_is_f64 == (form == MV_FORM_C128),
+             "mv_zmat_fused: the beam must be c128 for the c128 form and c64 "
+             "otherwise");
+  MV_REQUIRE(nterms < 2 || z1, "mv_zmat_fused: missing term array 1");
+  MV_REQUIRE(nterms < 3 || z2, "mv_zmat_fused: missing term array 2");
+  if (count == 0) return MV_OK;
+  const int apb = 16;
+  dim3 g((unsigned)cdiv(cdiv(count, kZPad) * kZPad, kThreads),
+         (unsigned)cdiv(nant, apb));
+  hipStream_t st = as_stream(stream);
+  const double fc = freq / kC;
+  ZOut zo = {{z0, z1, z2}};
+#define MV_FUSED(F, NT)                                                      \
+  return launch_fused<F, NT>(g, st, antpos, nant, fc, crd, ld_crd, flux,     \
+                             idx, beam, ld_beam, nbeam, beam_idx, nfeed, nax,\
+                             count, n_dev, scale, zo, ldz, seg_stride, apb)
+  switch (form) {
+    case MV_FORM_C128: MV_FUSED(MV_FORM_C128, 1);
+    case MV_FORM_C64: MV_FUSED(MV_FORM_C64, 1);
+    case MV_FORM_SPLIT_BF16:
+      if (nterms == 2) MV_FUSED(MV_FORM_SPLIT_BF16, 2);
+      MV_FUSED(MV_FORM_SPLIT_BF16, 3);
+    default:
+      if (nterms == 2) MV_FUSED(MV_FORM_SPLIT_F16, 2);
+      MV_FUSED(MV_FORM_SPLIT_F16, 3);
+  }
+#undef MV_FUSED
+}
+
+extern "C" int mv_split_z(const void *z, int z_is_f64, int64_t rows, int nseg,
+                          int64_t cols, int64_t ldz, int64_t seg_stride,
+                          int form, int nterms, double scale, void *t0,
+                          void *t1, void *t2, int64_t ldt,
+                          int64_t tseg_stride, mv_stream_t stream) {
+  MV_REQUIRE(z && t0, "mv_split_z: null pointer");
+  MV_REQUIRE(form == MV_FORM_SPLIT_BF16 || form == MV_FORM_SPLIT_F16,
+             "mv_split_z: form must be a split form");
+  MV_REQUIRE(nterms == 2 || nterms == 3, "mv_split_z: nterms must be 2 or 3");
+  MV_REQUIRE(nterms < 3 || (t1 && t2), "mv_split_z: missing term arrays");
+  MV_REQUIRE(t1, "mv_split_z: missing term array 1");
+  MV_REQUIRE(rows >= 0 && nseg >= 1 && cols >= 0 && rows * nseg <= 65535 &&
+                 seg_stride >= cols && ldz >= nseg * seg_stride &&
+                 tseg_stride >= cols && ldt >= nseg * tseg_stride,
+             "mv_split_z: bad sizes");
+  if (rows == 0 || cols == 0) return MV_OK;
+  dim3 g((unsigned)cdiv(cols, kThreads), (unsigned)(rows * nseg));
+  hipStream_t st = as_stream(stream);
+  ZOut to = {{t0, t1, t2}};
+  float sc = (float)scale;
+#define MV_SPLIT(TI, F, NT)                                                  \
+  split_kernel<TI, F, NT><<<g, kThreads, 0, st>>>(                           \
+      (const cplx<TI> *)z, rows, nseg, cols, ldz, seg_stride, sc, to, ldt,   \
+      tseg_stride)
+  if (form == MV_FORM_SPLIT_BF16) {
+    if (z_is_f64) {
+      if (nterms == 2) MV_SPLIT(double, MV_FORM_SPLIT_BF16, 2);
+      else MV_SPLIT(double, MV_FORM_SPLIT_BF16, 3);
+    } else {
+      if (nterms == 2) MV_SPLIT(float, MV_FORM_SPLIT_BF16, 2);
+      else MV_SPLIT(float, MV_FORM_SPLIT_BF16, 3);
+    }
+  } else {
+    if (z_is_f64) {
+      if (nterms == 2) MV_SPLIT(double, MV_FORM_SPLIT_F16, 2);
+      else MV_SPLIT(double, MV_FORM_SPLIT_F16, 3);
+    } else {
+      if (nterms == 2) MV_SPLIT(float, MV_FORM_SPLIT_F16, 2);
+      else MV_SPLIT(float, MV_FORM_SPLIT_F16, 3);
+    }
+  }
+#undef MV_SPLIT
+  return check_launch("mv_split_z");
+}

@@ -1,0 +1,156 @@
+"""``simulate``: the GPU visibility simulator (matvis gpu/gpu.py:57-256).
+
+Same keyword signature as the reference, plus ``split_terms`` /
+``split_products`` for the mixed-precision HERK forms.  The per-time loop
+runs as the fused :class:`~matvis_amd.gpu.engine.VisibilityEngine`.
+"""
+
+from __future__ import annotations
+
+import logging
+
+import numpy as np
+
+from .. import _device as D
+from .._utils import get_desired_chunks, get_dtypes
+from ..beams import unwrap
+from ..core import _validate_inputs
+from ..core.beams import _wrangle_beams
+from ..core.coords import CoordinateRotation
+from . import coords as _coords  # noqa: F401  (registers rotation methods)
+from . import matprod as mp
+from .beams import spline_order
+from .engine import VisibilityEngine
+
+logger = logging.getLogger(__name__)
+
+HAVE_CUDA = True
+
+
+def _resolve_matprod(matprod_method: str, precision: int, split_terms,
+                     split_products):
+    cls = getattr(mp, matprod_method)
+    form = cls.split_form
+    if form is None:
+        from .._lib import FORM_C64, FORM_C128
+
+        form = FORM_C128 if precision == 2 else FORM_C64
+        return form, 1, 1
+    return (form, split_terms or cls.default_terms,
+            split_products or cls.default_products)
+
+
+def simulate_multi(
+    *,
+    antpos,
+    freqs,
+    times,
+    skycoords,
+    telescope_loc,
+    fluxes,
+    beam_list,
+    polarized: bool = False,
+    antpairs=None,
+    beam_idx=None,
+    max_memory=np.inf,
+    min_chunks: int = 1,
+    precision: int = 1,
+    beam_spline_opts: dict | None = None,
+    coord_method: str = "CoordinateRotationAstropy",
+    matprod_method: str = "GPUMatMul",
+    source_buffer: float = 1.0,
+    coord_method_params: dict | None = None,
+    memory_buffer: float = 0.9,
+    split_terms: int | None = None,
+    split_products: int | None = None,
+):
+    """All frequencies at once: returns (nfreq, ntimes, npairs, nfeed, nfeed).
+
+    ``fluxes`` is (NSRCS, NFREQS).  Equivalent to calling :func:`simulate`
+    once per frequency (wrapper.py:180-199), with the rotation shared.
+    """
+    if not 0 < source_buffer <= 1:
+        raise ValueError("source_buffer must satisfy 0 < source_buffer <= 1")
+    if not 0 < memory_buffer <= 1:
+        raise ValueError("memory_buffer must satisfy 0 < memory_buffer <= 1")
+    freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
+    fluxes = np.asarray(fluxes, dtype=np.float64)
+    if fluxes.ndim == 1:
+        fluxes = fluxes[:, None]
+    nax, nfeed, nant, ntimes = _validate_inputs(
+        precision, polarized, np.asarray(antpos), times, fluxes[:, 0]
+    )
+    rtype, ctype = get_dtypes(precision)
+    nsrc = fluxes.shape[0]
+    form, nterms, nprod = _resolve_matprod(matprod_method, precision,
+                                           split_terms, split_products)
+    freemem = min(max_memory, D.free_memory())
+    inner_beams = [unwrap(b)[0] for b in beam_list]
+    nchunks, npixc = get_desired_chunks(
+        freemem, min_chunks, inner_beams, nax, nfeed, nant, nsrc, precision,
+        source_buffer=source_buffer, memory_buffer=memory_buffer,
+    )
+    nsrc_alloc = int(npixc * source_buffer)
+    beams, _, beam_idx = _wrangle_beams(beam_idx, beam_list, polarized, nant,
+                                        float(freqs[0]))
+    coord_cls = CoordinateRotation._methods[coord_method]
+    coords = coord_cls(
+        flux=np.sqrt(0.5 * fluxes[:, 0]), times=times,
+        telescope_loc=telescope_loc, skycoords=skycoords, chunk_size=npixc,
+        precision=precision, source_buffer=source_buffer, gpu=True,
+        **(coord_method_params or {}),
+    )
+    coords.setup()
+    engine = VisibilityEngine(
+        antpos=antpos, freqs=freqs, coords=coords, fluxes=fluxes,
+        beam_list=beams, beam_idx=beam_idx, polarized=polarized,
+        precision=precision, form=form, nterms=nterms, nprod=nprod,
+        order=spline_order(beam_spline_opts), nchunks=nchunks,
+        chunk_size=npixc, nsrc_alloc=nsrc_alloc, antpairs=antpairs,
+    )
+    logger.info(f"Running with {nchunks} chunks, form {form}")
+    engine.setup()
+    return engine.run(ntimes)
+
+
+def simulate(
+    *,
+    antpos,
+    freq: float,
+    times,
+    skycoords,
+    telescope_loc,
+    I_sky,
+    beam_list,
+    polarized: bool = False,
+    antpairs=None,
+    beam_idx=None,
+    max_memory=np.inf,
+    min_chunks: int = 1,
+    precision: int = 1,
+    beam_spline_opts: dict | None = None,
+    coord_method: str = "CoordinateRotationAstropy",
+    matprod_method: str = "GPUMatMul",
+    source_buffer: float = 1.0,
+    coord_method_params: dict | None = None,
+    memory_buffer: float = 0.9,
+    split_terms: int | None = None,
+    split_products: int | None = None,
+) -> np.ndarray:
+    """GPU implementation of the visibility simulator at one frequency.
+
+    Returns ``(NTIMES, NBLS, NFEED, NFEED)`` if polarized, else
+    ``(NTIMES, NBLS)`` (gpu/gpu.py:256), complex64 or complex128 by precision.
+    """
+    vis = simulate_multi(
+        antpos=antpos, freqs=[freq], times=times, skycoords=skycoords,
+        telescope_loc=telescope_loc, fluxes=np.asarray(I_sky)[:, None],
+        beam_list=beam_list, polarized=polarized, antpairs=antpairs,
+        beam_idx=beam_idx, max_memory=max_memory, min_chunks=min_chunks,
+        precision=precision, beam_spline_opts=beam_spline_opts,
+        coord_method=coord_method, matprod_method=matprod_method,
+        source_buffer=source_buffer, coord_method_params=coord_method_params,
+        memory_buffer=memory_buffer, split_terms=split_terms,
+        split_products=split_products,
+    )[0]
+    return vis if polarized else vis[:, :, 0, 0]

@@ -1,0 +1,359 @@
+"""CPU oracle for matvis's per-time visibility computation.
+
+TEST INFRASTRUCTURE ONLY.  The product package (``matvis_amd``) never imports
+this module.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` use it: as the checker the HIP path is
+compared against, and as the CPU baseline timed beside the GPU figures.
+
+It is a numpy (float64 / complex128) restatement of matvis's CPU path at
+``precision=2``.  Every function cites the reference file:line it follows
+(paths relative to the matvis source tree).  The reference itself cannot be
+imported here (it hard-imports pyuvdata, which is absent), so the restatement is
+pinned instead by:
+
+* the reference's own known-answer tests, ported with the same inputs
+  (``tests/test_matprod.py``, ``tests/test_cublas.py``,
+  ``tests/test_beam_interp_gpu.py``, ``tests/test_coordinates.py``,
+  ``tests/test_matvis_gpu.py::test_antizenith``);
+* ``scipy.ndimage.map_coordinates`` (the routine the reference's CPU beam path
+  reaches through pyuvdata, and that cupyx mirrors on its GPU path), which the
+  beam interpolation here calls directly;
+* closed forms and brute force on tiny inputs (zenith phase = 1, Hermitian
+  symmetry, an explicit triple loop for ``conj(Z) Z^T`` with the feed swap).
+
+Parity status per function (see ``tests/test_oracle.py``):
+
+==========================  =====================================================
+``point_source_crd_eq``     pinned: cardinal points (test_coordinates.py:21-68)
+``eci_to_enu_matrix``       pinned: zenith at HA=0, orthonormal (:71-106)
+``rotate``                  pinned: brute force vs an explicit matrix product
+``select_chunk``            pinned: ascending-order compaction, zero padding
+``enu_to_az_za``            pinned: E/N/U conventions (test_coordinates.py:108-160)
+``interp_grid_beam``        pinned: scipy map_coordinates; node identity and
+                            linear exactness (test_beam_interp_gpu.py:12-64)
+``gaussian_beam``           **parity unpinned**: restates pyuvdata's
+                            GaussianBeam e-field ``exp(-za^2 / (2 sigma^2))``
+                            (and ``/sqrt(2)`` per feed/axis when polarized);
+                            pyuvdata is absent, nothing checks the formula
+``exptau``                  pinned: zenith source gives exactly 1; closed form
+                            ``exp(+2 pi i nu b.s / c)`` on a 1-antenna case
+``zmatrix``                 pinned: per-antenna beam loop == broadcast
+                            (test_matvis_cpu.py:61-74 bit-exact invariance)
+``vis_from_z``              pinned: test_matprod.py:14-71 (arange + 1j),
+                            test_cublas.py:13-25, explicit triple loop
+``simulate``                pinned: anti-zenith source gives V == 0
+                            (test_matvis_gpu.py:15-25); chunk-count invariance
+==========================  =====================================================
+"""
+
+from __future__ import annotations
+
+import numpy as np
+from scipy import ndimage
+
+#: Speed of light in m/s, the value of ``astropy.constants.c`` used by the
+#: reference (core/tau.py:4,31).
+C_LIGHT = 299792458.0
+
+
+def get_dtypes(precision: int):
+    """(real, complex) dtypes for a precision level (_utils.py:291-319)."""
+    if precision == 1:
+        return np.float32, np.complex64
+    return np.float64, np.complex128
+
+
+def point_source_crd_eq(ra, dec):
+    """ECI unit vectors ``(cos ra cos dec, cos dec sin ra, sin dec)``.
+
+    Follows coordinates.py:174-200.  Shape ``(3, nsrc)``.
+    """
+    ra = np.asarray(ra, dtype=np.float64)
+    dec = np.asarray(dec, dtype=np.float64)
+    return np.asarray(
+        [np.cos(ra) * np.cos(dec), np.cos(dec) * np.sin(ra), np.sin(dec)]
+    )
+
+
+def eci_to_enu_matrix(ha, lat):
+    """3x3 ECI -> ENU rotation at hour angle ``ha`` and latitude ``lat``.
+
+    Follows coordinates.py:91-128.
+    """
+    return np.array(
+        [
+            [-np.sin(ha), np.cos(ha), 0.0 * ha],
+            [-np.sin(lat) * np.cos(ha), -np.sin(lat) * np.sin(ha), np.cos(lat)],
+            [np.cos(lat) * np.cos(ha), np.cos(lat) * np.sin(ha), np.sin(lat)],
+        ]
+    )
+
+
+def rotate(rot, eci):
+    """Topocentric ENU vectors ``R_t . X_eq``.
+
+    Step 2 of docs/understanding_the_algorithm.rst:118-121 (the ERFA path's
+    ``_atioq`` matmul, cpu/coords.py:89-116).  Each output row is evaluated as
+    ``(R[i,0]*x + R[i,1]*y) + R[i,2]*z`` in float64, one rounding per
+    operation and no fused multiply-add, so that the horizon test ``z > 0``
+    is reproducible bit for bit by any implementation that keeps this order.
+    """
+    rot = np.asarray(rot, dtype=np.float64)
+    x, y, z = np.asarray(eci, dtype=np.float64)
+    out = np.empty((3, x.size), dtype=np.float64)
+    for i in range(3):
+        out[i] = rot[i, 0] * x + rot[i, 1] * y + rot[i, 2] * z
+    return out
+
+
+def select_chunk(topo, flux, chunk: int, chunk_size: int, nsrc_alloc: int):
+    """Horizon cull and stable compaction of one source chunk.
+
+    Follows core/coords.py:99-150: ``where(topo[2] > 0)`` keeps ascending
+    source order; coordinates and fluxes are gathered into buffers of length
+    ``nsrc_alloc``; fluxes past ``n`` are zero.  Coordinates past ``n`` are
+    zero here (the reference leaves stale values there, which its zero flux
+    makes irrelevant).  Returns ``(crd, flux, n, index)`` where ``index`` holds
+    the global source indices kept.
+    """
+    slc = slice(chunk * chunk_size, (chunk + 1) * chunk_size)
+    t = topo[:, slc]
+    f = flux[slc]
+    above = np.where(t[2] > 0)[0]
+    n = len(above)
+    if n > nsrc_alloc:
+        raise ValueError(
+            f"nsrc_alloc ({nsrc_alloc}) is too small for the number of "
+            f"sources above horizon ({n}). Try increasing source_buffer."
+        )
+    crd = np.zeros((3, nsrc_alloc), dtype=np.float64)
+    fl = np.zeros(nsrc_alloc, dtype=f.dtype)
+    crd[:, :n] = t[:, above]
+    fl[:n] = f[above]
+    return crd, fl, n, above + chunk * chunk_size
+
+
+def enu_to_az_za(enu_e, enu_n, orientation="astropy", periodic_azimuth=True):
+    """Direction cosines -> (azimuth, zenith angle).
+
+    Follows coordinates.py:31-88, including the uvbeam orientation flip
+    ``az -> pi/2 - az`` and the ``mod 2 pi``.
+    """
+    assert orientation in ("astropy", "uvbeam")
+    enu_e = np.asarray(enu_e, dtype=np.float64)
+    enu_n = np.asarray(enu_n, dtype=np.float64)
+    lsqr = enu_n**2.0 + enu_e**2.0
+    mask = lsqr < 1
+    zeta = np.zeros_like(lsqr)
+    zeta[mask] = np.sqrt(1 - lsqr[mask])
+    az = np.arctan2(enu_e, enu_n)
+    za = 0.5 * np.pi - np.arcsin(zeta)
+    if orientation == "uvbeam":
+        az = 0.5 * np.pi - az
+    if periodic_azimuth:
+        az = np.mod(az, 2 * np.pi)
+    return az, za
+
+
+def interp_grid_beam(planes, daz, dza, azmin, az, za, order: int = 1):
+    """Interpolate az/za grid beams at source positions.
+
+    Follows gpu/beams.py:127-200 (``gpu_beam_interpolation``): ``planes`` has
+    shape ``(nbeam, nax, nfeed, nza, naz)`` and is real (power beam) or complex
+    (E-field beam); the map coordinates are ``[za / dza, (az - azmin) / daz]``
+    (gpu/beams.py:185); ``map_coordinates`` runs with its defaults
+    (``mode='constant'``, ``cval=0``, spline prefilter for ``order > 1``); a
+    power beam ends with a square root taken in the complex output buffer
+    (gpu/beams.py:195-197 with ``out`` complex, gpu/beams.py:122), so a
+    negative interpolated power ``p`` becomes ``1j * sqrt(-p)``.
+
+    Returns ``(nbeam, nfeed, nax, nsrc)`` complex128.
+    """
+    planes = np.asarray(planes)
+    nbeam, nax, nfeed = planes.shape[:3]
+    az = np.asarray(az, dtype=np.float64)
+    za = np.asarray(za, dtype=np.float64)
+    out = np.zeros((nbeam, nfeed, nax, az.size), dtype=np.complex128)
+    for bm in range(nbeam):
+        coords = np.asarray([za / dza[bm], (az - azmin[bm]) / daz[bm]])
+        for fd in range(nfeed):
+            for ax in range(nax):
+                out[bm, fd, ax] = ndimage.map_coordinates(
+                    planes[bm, ax, fd], coords, order=order
+                )
+    if not np.iscomplexobj(planes):
+        out = np.sqrt(out)
+    return out
+
+
+def gaussian_beam(sigma, za, polarized: bool):
+    """Analytic Gaussian beam response, shape ``(nbeam, nfeed, nax, nsrc)``.
+
+    Restates pyuvdata's ``GaussianBeam`` as the reference consumes it
+    (core/beams.py:15-32, cpu/beams.py:41-46): the E-field amplitude is
+    ``exp(-za^2 / (2 sigma^2))``; unpolarized simulations take the square
+    root of the xx power beam, which is that amplitude; polarized ones see
+    the amplitude divided by sqrt(2) on every (feed, axis).
+    PARITY UNPINNED: pyuvdata is absent, nothing checks this formula.
+    """
+    sigma = np.atleast_1d(np.asarray(sigma, dtype=np.float64))
+    za = np.asarray(za, dtype=np.float64)
+    nf = na = 2 if polarized else 1
+    out = np.zeros((sigma.size, nf, na, za.size), dtype=np.complex128)
+    for b, sg in enumerate(sigma):
+        e = np.exp(-(za**2) / (2.0 * sg**2))
+        out[b] = e / np.sqrt(2.0) if polarized else e
+    return out
+
+
+def exptau(antpos, freq: float, crd):
+    """``exp(i 2 pi nu (b . s) / c)``, shape ``(nant, nsrc)``.
+
+    Follows core/tau.py:30-31 (``antpos * 2 pi nu * 1j / c``, positive sign)
+    and core/tau.py:65-66 (matmul then exp).
+    """
+    ang_freq = np.float64(2.0 * np.pi * freq)
+    antpos_c = np.asarray(antpos, dtype=np.float64) * ang_freq * 1j / C_LIGHT
+    return np.exp(antpos_c @ np.asarray(crd, dtype=np.float64))
+
+
+def zmatrix(sqrt_flux, beam, exptau_, beam_idx, nfeed: int, nax: int):
+    """``Z = A sqrt(I) exp(tau)``, shape ``(nant * nfeed, nax * nsrc)``.
+
+    Follows core/getz.py:80-100: row = ``ant * nfeed + feed``, column =
+    ``ax * nsrc + src``; ``beam`` is ``(nbeam, nfeed, nax, nsrc)`` broadcast
+    over antennas, or indexed per antenna through ``beam_idx``.
+    """
+    nant, nsrc = exptau_.shape
+    e = exptau_ * sqrt_flux
+    z = np.empty((nant, nfeed, nax, nsrc), dtype=np.complex128)
+    z[:] = e[:, None, None, :]
+    if beam_idx is None:
+        z *= beam
+    else:
+        for ant, b in enumerate(beam_idx):
+            z[ant] *= beam[b]
+    return z.reshape(nant * nfeed, nax * nsrc)
+
+
+def vis_from_z(z, nant: int, nfeed: int, antpairs=None):
+    """``V = conj(Z) Z^T`` in the API layout ``(npairs, nfeed, nfeed)``.
+
+    Follows cpu/matprod.py:21-30: the product is reshaped to
+    ``(nant, nfeed, nant, nfeed)`` and transposed ``(0, 2, 3, 1)``, so that
+    ``out[i * nant + j, p, q] = sum conj(Z[i*nf + q]) Z[j*nf + p]`` (the feed
+    swap that tests/test_matprod.py:56-65 pins).
+    """
+    v = z.conj().dot(z.T)
+    v = v.reshape(nant, nfeed, nant, nfeed).transpose(0, 2, 3, 1)
+    if antpairs is None:
+        return v.reshape(nant * nant, nfeed, nfeed)
+    antpairs = np.asarray(antpairs)
+    return v[antpairs[:, 0], antpairs[:, 1]]
+
+
+def desired_chunks(nsrc: int, nchunks: int):
+    """``(nchunks, sources per chunk)`` as get_desired_chunks returns them.
+
+    Follows _utils.py:259-288: ``nchunks = min(nchunks, nsrc)`` and
+    ``ceil(nsrc / nchunks)`` sources per chunk.
+    """
+    nchunks = max(1, min(nchunks, nsrc))
+    return nchunks, int(np.ceil(nsrc / nchunks))
+
+
+def beam_response(beam: dict, crd, polarized: bool):
+    """Evaluate a beam description at compacted topocentric coordinates.
+
+    ``beam`` is ``{"kind": "grid", "planes", "daz", "dza", "azmin",
+    "order"}`` or ``{"kind": "gaussian", "sigma"}``.  Grid beams go through
+    ``enu_to_az_za(..., orientation="uvbeam")`` first (gpu/beams.py:110).
+    """
+    az, za = enu_to_az_za(crd[0], crd[1], orientation="uvbeam")
+    if beam["kind"] == "grid":
+        return interp_grid_beam(
+            beam["planes"], beam["daz"], beam["dza"], beam["azmin"], az, za,
+            order=beam.get("order", 1),
+        )
+    if beam["kind"] == "gaussian":
+        return gaussian_beam(beam["sigma"], za, polarized)
+    raise ValueError(f"unknown beam kind {beam['kind']!r}")
+
+
+def simulate(
+    *,
+    antpos,
+    freq: float,
+    eci,
+    rotations,
+    I_sky,
+    beam: dict,
+    polarized: bool = False,
+    antpairs=None,
+    beam_idx=None,
+    nchunks: int = 1,
+    source_buffer: float = 1.0,
+):
+    """The per-time loop of ``cpu.simulate`` at precision 2.
+
+    Follows cpu/cpu.py:151-277: flux ``sqrt(0.5 I)`` (cpu/cpu.py:188 via
+    gpu/gpu.py:127), per time a rotation, per chunk a cull/compaction, beam,
+    exp(tau), Z and ``conj(Z) Z^T``; the chunk partials are summed
+    (core/matprod.py:103-115).  A chunk with no source above the horizon
+    contributes exact zeros (the CPU path has no ``nsrcs_up < 1`` skip).
+    ``rotations`` is ``(ntimes, 3, 3)``: the host-computed ``R_t`` per time
+    that the device path applies too.  ``nsrc_alloc`` follows the GPU rule
+    ``int(chunk * source_buffer)`` (gpu/gpu.py:112).
+
+    Returns ``(ntimes, npairs, nfeed, nfeed)`` if polarized, else
+    ``(ntimes, npairs)``, complex128.
+    """
+    I_sky = np.asarray(I_sky, dtype=np.float64)
+    assert I_sky.ndim == 1, "I_sky must have shape (NSRCS,)."
+    nax = nfeed = 2 if polarized else 1
+    antpos = np.asarray(antpos, dtype=np.float64)
+    nant = antpos.shape[0]
+    nsrc = I_sky.size
+    nchunks, chunk_size = desired_chunks(nsrc, nchunks)
+    nsrc_alloc = int(chunk_size * source_buffer)
+    flux = np.sqrt(0.5 * I_sky)
+    rotations = np.asarray(rotations, dtype=np.float64)
+    ntimes = rotations.shape[0]
+    npairs = nant * nant if antpairs is None else len(antpairs)
+    vis = np.zeros((ntimes, npairs, nfeed, nfeed), dtype=np.complex128)
+    for t in range(ntimes):
+        topo = rotate(rotations[t], eci)
+        for c in range(nchunks):
+            crd, fl, _, _ = select_chunk(topo, flux, c, chunk_size, nsrc_alloc)
+            a = beam_response(beam, crd, polarized)
+            et = exptau(antpos, freq, crd)
+            z = zmatrix(fl, a, et, beam_idx, nfeed, nax)
+            vis[t] += vis_from_z(z, nant, nfeed, antpairs)
+    return vis if polarized else vis[:, :, 0, 0]
+
+
+def normalized_error(v_test, v_ref, nant: int, nfeed: int):
+    """Max and median of ``|V - V_ref| / sqrt(|V_ref_ii| |V_ref_jj|)``.
+
+    ``v_*`` are ``(..., nant*nant, nfeed, nfeed)`` in the API layout.  The
+    normaliser of element ``(i, j, p, q)`` is the geometric mean of the two
+    autocorrelations it pairs, ``V[i,i,q,q]`` and ``V[j,j,p,p]``.  Returns
+    ``(max, median, per-element max relative, per-element median relative)``.
+    """
+    vt = np.asarray(v_test).reshape(-1, nant, nant, nfeed, nfeed)
+    vr = np.asarray(v_ref).reshape(-1, nant, nant, nfeed, nfeed)
+    idx = np.arange(nant)
+    auto = np.abs(vr[:, idx, idx])  # (T, nant, nfeed, nfeed)
+    autod = np.stack([auto[:, :, f, f] for f in range(nfeed)], axis=-1)
+    # element (i, j, p, q) pairs conj(Z[i,q]) with Z[j,p]
+    norm = np.sqrt(autod[:, :, None, None, :] * autod[:, None, :, :, None])
+    diff = np.abs(vt - vr)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        nerr = np.where(norm > 0, diff / norm, 0.0)
+        rel = np.where(np.abs(vr) > 0, diff / np.abs(vr), 0.0)
+    return (
+        float(nerr.max()),
+        float(np.median(nerr)),
+        float(rel.max()),
+        float(np.median(rel)),
+    )

@@ -1,0 +1,239 @@
+"""End-to-end parity of the drop-in entry points against the CPU oracle.
+
+The oracle and the device share the per-time rotations R_t (passed through
+``CoordinateRotationMatrix``), so every difference is the per-time pipeline's.
+Error metric: |V - V_ref| / sqrt(|V_ref_ii| |V_ref_jj|), max over baselines,
+feeds and times.  Bounds: FP64 plain 1e-12, complex64 1e-5, split forms as
+stated.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from _problems import small_problem
+from matvis_amd import astro
+from matvis_amd.beams import GaussianBeam
+from matvis_amd.gpu.gpu import simulate, simulate_multi
+
+pytestmark = pytest.mark.gpu
+
+
+def run_gpu(p, **kw):
+    opts = dict(antpos=p["antpos"], freq=p["freq"], times=p["times"],
+                skycoords=astro.SkyCoord(p["ra"], p["dec"]),
+                telescope_loc=p["loc"], I_sky=p["I_sky"],
+                beam_list=p["beam_list"], polarized=p["polarized"],
+                coord_method="CoordinateRotationMatrix",
+                coord_method_params={"rotations": p["rotations"]},
+                beam_spline_opts={"order": p["order"]}, precision=2)
+    opts.update(kw)
+    return simulate(**opts)
+
+
+def run_oracle(p, **kw):
+    opts = dict(antpos=p["antpos"], freq=p["freq"], eci=p["eci"],
+                rotations=p["rotations"], I_sky=p["I_sky"],
+                beam=p["oracle_beam"], polarized=p["polarized"])
+    opts.update(kw)
+    return O.simulate(**opts)
+
+
+def nerr(v, ref, nant, polarized):
+    nf = 2 if polarized else 1
+    if not polarized:
+        v, ref = v[..., None, None], ref[..., None, None]
+    return O.normalized_error(v, ref, nant, nf)
+
+
+@pytest.mark.parametrize("polarized", [False, True])
+def test_gaussian_fp64_matches_oracle(polarized):
+    """Config 0 scaled to 3 times: 16 antennas, 2000 sources."""
+    p = small_problem(nant=16, nsrc=2000, ntimes=3, polarized=polarized)
+    v = run_gpu(p)
+    ref = run_oracle(p)
+    assert v.dtype == np.complex128 and v.shape == ref.shape
+    e = nerr(v, ref, 16, polarized)
+    print(f"gaussian fp64 pol={polarized}: normalised max {e[0]:.2e}, "
+          f"per-element max {e[2]:.2e}", flush=True)
+    assert e[0] < 1e-12
+
+
+@pytest.mark.parametrize("order", [1, 3])
+def test_grid_beam_fp64_matches_oracle(order):
+    p = small_problem(nant=12, nsrc=1500, ntimes=2, polarized=True,
+                      beam="grid", order=order)
+    e = nerr(run_gpu(p), run_oracle(p), 12, True)
+    print(f"grid order={order}: normalised max {e[0]:.2e}", flush=True)
+    assert e[0] < 1e-12
+
+
+def test_multichunk_source_buffer_and_antpairs():
+    p = small_problem(nant=10, nsrc=3001, ntimes=2, polarized=True)
+    pairs = np.array([(0, 0), (3, 7), (7, 3), (9, 1), (2, 2)])
+    v = run_gpu(p, min_chunks=3, source_buffer=0.8, antpairs=pairs)
+    ref = run_oracle(p, nchunks=3, source_buffer=0.8, antpairs=pairs)
+    np.testing.assert_allclose(v, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    v1 = run_gpu(p, min_chunks=1, antpairs=pairs)
+    np.testing.assert_allclose(v1, v, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def test_source_buffer_overflow_raises():
+    p = small_problem(nant=4, nsrc=3000, ntimes=1)
+    with pytest.raises(ValueError, match="is too small for the number of"):
+        run_gpu(p, source_buffer=0.3)
+
+
+def test_antizenith_is_exact_zero():
+    """tests/test_matvis_gpu.py:15-25."""
+    p = small_problem(nant=3, nsrc=1, ntimes=2, polarized=True)
+    lat = np.radians(-30.72)
+    p["ra"], p["dec"] = np.array([0.0]), np.array([np.pi / 2 + lat - 0.2])
+    p["I_sky"] = np.array([2.0])
+    assert np.all(run_gpu(p) == 0)
+
+
+def test_multibeam_invariances():
+    """tests/test_matvis_gpu.py:28-110: per-antenna copies == one beam;
+    permuted beam list with remapped beam_idx is the same; a different
+    assignment differs."""
+    p = small_problem(nant=3, nsrc=400, ntimes=1)
+    ga, gb = GaussianBeam(sigma=0.2), GaussianBeam(sigma=0.12)
+    v1 = run_gpu(p, beam_list=[ga])
+    v2 = run_gpu(p, beam_list=[ga] * 3, beam_idx=np.zeros(3, int))
+    assert np.array_equal(v1, v2)
+    vab = run_gpu(p, beam_list=[ga, gb], beam_idx=np.array([0, 1, 0]))
+    vba = run_gpu(p, beam_list=[gb, ga], beam_idx=np.array([1, 0, 1]))
+    assert np.array_equal(vab, vba)
+    v101 = run_gpu(p, beam_list=[ga, gb], beam_idx=np.array([1, 0, 1]))
+    assert not np.allclose(vab, v101, rtol=0, atol=1e-12)
+    ref = run_oracle(p, beam={"kind": "gaussian", "sigma": [0.2, 0.12]},
+                     beam_idx=np.array([0, 1, 0]))
+    assert nerr(vab, ref, 3, False)[0] < 1e-12
+
+
+def test_mixed_beams_raise():
+    from matvis_amd import synth
+
+    p = small_problem(nant=3, nsrc=10, ntimes=1)
+    grid = synth.harmonic_efield_beam(1, naz=36, nza=10)
+    with pytest.raises(ValueError, match="GPUBeamInterpolator only supports"):
+        run_gpu(p, beam_list=[grid, GaussianBeam(sigma=0.1), grid])
+
+
+def test_single_precision_matches_oracle():
+    """tests/test_matvis_gpu.py:127-139 style: precision 1 within 1e-5."""
+    p = small_problem(nant=12, nsrc=1500, ntimes=2, polarized=True, beam="grid")
+    v = run_gpu(p, precision=1)
+    assert v.dtype == np.complex64
+    e = nerr(v, run_oracle(p), 12, True)
+    print(f"precision 1: normalised max {e[0]:.2e}", flush=True)
+    assert e[0] < 1e-5
+
+
+@pytest.mark.parametrize("method,terms,prods,bound", [
+    ("GPUMatMulSplitBF16", 2, 4, 1e-6), ("GPUMatMulSplitBF16", 2, 3, 1e-5),
+    ("GPUMatMulSplitBF16", 3, 6, 1e-6), ("GPUMatMulSplitF16", 2, 3, 1e-6)])
+def test_split_forms_match_oracle(method, terms, prods, bound):
+    p = small_problem(nant=20, nsrc=4000, ntimes=2, polarized=True, beam="grid",
+                      order=3)
+    v = run_gpu(p, matprod_method=method, split_terms=terms,
+                split_products=prods)
+    e = nerr(v, run_oracle(p), 20, True)
+    print(f"{method} ({terms},{prods}): normalised max {e[0]:.2e} median "
+          f"{e[1]:.2e}", flush=True)
+    assert e[0] < bound
+
+
+def test_simulate_vis_frequencies_match_single_calls():
+    """wrapper.py:180-199: the frequency loop; here rotations are shared."""
+    import matvis_amd
+
+    p = small_problem(nant=6, nsrc=500, ntimes=2, polarized=True)
+    freqs = np.array([120e6, 150e6, 180e6])
+    fluxes = p["I_sky"][:, None] * (freqs[None] / 150e6) ** -0.8
+    ants = {i: tuple(p["antpos"][i]) for i in range(6)}
+    kw = dict(coord_method="CoordinateRotationMatrix",
+              coord_method_params={"rotations": p["rotations"]})
+    vis = matvis_amd.simulate_vis(ants, fluxes, p["ra"], p["dec"], freqs,
+                                  p["times"], p["beam_list"], p["loc"],
+                                  polarized=True, precision=2, use_gpu=True,
+                                  **kw)
+    assert vis.shape == (3, 2, 36, 2, 2)
+    for i, f in enumerate(freqs):
+        single = run_gpu(p, freq=f, I_sky=fluxes[:, i])
+        assert np.array_equal(vis[i], single)
+        ref = run_oracle(p, freq=f, I_sky=fluxes[:, i])
+        assert nerr(vis[i], ref, 6, True)[0] < 1e-12
+
+
+def test_plugin_loop_matches_oracle():
+    """Drive the plugin classes as gpu.simulate's loop does
+    (gpu/gpu.py:197-254) and compare with the oracle."""
+    from matvis_amd.core.coords import CoordinateRotation
+    from matvis_amd.core.getz import ZMatrixCalc
+    from matvis_amd.core.tau import TauCalculator
+    from matvis_amd.gpu import coords as _  # noqa: F401
+    from matvis_amd.gpu.beams import GPUBeamInterpolator
+    from matvis_amd.gpu.matprod import GPUMatMul
+
+    p = small_problem(nant=8, nsrc=1200, ntimes=2, polarized=True, beam="grid")
+    nchunks, chunk = 2, 600
+    co = CoordinateRotation._methods["CoordinateRotationMatrix"](
+        rotations=p["rotations"], flux=np.sqrt(0.5 * p["I_sky"]),
+        times=p["times"], telescope_loc=p["loc"],
+        skycoords=astro.SkyCoord(p["ra"], p["dec"]), chunk_size=chunk,
+        precision=2, source_buffer=1.0)
+    bm = GPUBeamInterpolator(beam_list=p["beam_list"], beam_idx=None,
+                             polarized=True, nant=8, freq=p["freq"],
+                             nsrc=co.nsrc_alloc, precision=2)
+    tau = TauCalculator(antpos=p["antpos"], freq=p["freq"], nsrc=co.nsrc_alloc,
+                        precision=2)
+    zc = ZMatrixCalc(nant=8, nfeed=2, nax=2, nsrc=co.nsrc_alloc,
+                     ctype=np.complex128)
+    mp = GPUMatMul(nchunks, 2, 8, None, precision=2)
+    for obj in (tau, bm, co, zc, mp):
+        obj.setup()
+    vis = np.zeros((2, 64, 2, 2), np.complex128)
+    for t in range(2):
+        co.rotate(t)
+        for c in range(nchunks):
+            crd, isq, n = co.select_chunk(c, t)
+            a = bm(crd[0], crd[1], check=t == 0)
+            et = tau(crd)
+            z = zc(isq, a, et, bm.beam_idx)
+            mp(z, c)
+        mp.sum_chunks(vis[t])
+    ref = run_oracle(p, nchunks=2)
+    assert nerr(vis, ref, 8, True)[0] < 1e-12
+    # all_coords_topo is the full rotation of every source
+    np.testing.assert_array_equal(
+        co.all_coords_topo.cpu().numpy(), O.rotate(p["rotations"][1], p["eci"]))
+
+
+def test_nan_beam_raises():
+    """tests/test_beams.py:124-146: a NaN in the beam is reported."""
+    from matvis_amd.gpu.beams import GPUBeamInterpolator
+    from matvis_amd import synth
+
+    b = synth.harmonic_efield_beam(2, naz=36, nza=10, dtype=np.complex128)
+    b.data_array[1] = np.nan
+    tx = np.linspace(-0.7, 0.7, 100)
+    bm = GPUBeamInterpolator(beam_list=[b], beam_idx=np.zeros(1, int),
+                             polarized=True, nant=1, freq=150e6, nsrc=100,
+                             precision=2)
+    bm.setup()
+    with pytest.raises(ValueError, match="invalid value"):
+        bm(tx, tx, check=True)
+
+
+def test_simulate_multi_shapes_and_dtype():
+    p = small_problem(nant=5, nsrc=300, ntimes=2)
+    v = simulate_multi(antpos=p["antpos"], freqs=[1e8, 1.2e8], times=p["times"],
+                       skycoords=astro.SkyCoord(p["ra"], p["dec"]),
+                       telescope_loc=p["loc"],
+                       fluxes=np.stack([p["I_sky"]] * 2, 1),
+                       beam_list=p["beam_list"], precision=1,
+                       coord_method="CoordinateRotationERFA")
+    assert v.shape == (2, 2, 25, 1, 1) and v.dtype == np.complex64
+    assert np.isfinite(v).all()

@@ -1,0 +1,190 @@
+"""Host-side logic: chunk model, beam preparation, astrometry, synthetic
+inputs, plugin argument validation (no GPU needed)."""
+
+import math
+
+import numpy as np
+import pytest
+
+from matvis_amd import astro, synth
+from matvis_amd._utils import get_desired_chunks, get_required_chunks
+from matvis_amd.beams import BeamInterface, GaussianBeam, GridBeam, grid_planes
+from matvis_amd.core.beams import _wrangle_beams
+
+
+def test_required_chunks_monotone_and_validated():
+    """tests/test_utils.py:20-82."""
+    args = (2, 2, 350, 100_000, 1, 360 * 91, 2)
+    big = get_required_chunks(np.inf, *args)
+    assert big == 1
+    counts = [get_required_chunks(mem, *args) for mem in (2**34, 2**31, 2**29)]
+    assert counts == sorted(counts)
+    with pytest.raises(ValueError, match=r"memory_buffer must be in the range"):
+        get_required_chunks(1e9, *args, memory_buffer=1.5)
+    n, per = get_desired_chunks(np.inf, 3, [], 1, 1, 8, 10, 2)
+    assert (n, per) == (3, 4)
+    n, per = get_desired_chunks(np.inf, 50, [], 1, 1, 8, 10, 2)
+    assert (n, per) == (10, 1)
+
+
+def _grid_beam(naz=36, nza=10, full_circle=True):
+    az = np.radians(np.arange(naz) * (360.0 / naz if full_circle else 5.0))
+    za = np.radians(np.linspace(0, 90, nza))
+    rng = np.random.default_rng(0)
+    data = rng.normal(size=(2, 2, 1, nza, naz)) + 1j * rng.normal(size=(2, 2, 1, nza, naz))
+    return GridBeam(data, az, za)
+
+
+def test_grid_planes_wraps_azimuth_and_converts_power():
+    b = _grid_beam()
+    planes, daz, dza, azmin, is_power = grid_planes(b, 150e6, polarized=True)
+    assert planes.shape == (2, 2, 10, 37) and not is_power
+    np.testing.assert_array_equal(planes[..., -1], planes[..., 0])
+    assert math.isclose(daz, math.radians(10)) and azmin == 0.0
+    planes, *_, is_power = grid_planes(b, 150e6, polarized=False)
+    e = b.data_array[:, 0, 0]
+    np.testing.assert_allclose(planes[0, 0, :, :36], (np.abs(e) ** 2).sum(0))
+    assert is_power and planes.shape == (1, 1, 10, 37)
+
+
+def test_grid_planes_partial_azimuth_not_wrapped():
+    b = _grid_beam(naz=20, full_circle=False)
+    planes, *_ = grid_planes(b, 150e6, polarized=True)
+    assert planes.shape[-1] == 20
+
+
+def test_grid_planes_rejects_bad_grids():
+    b = _grid_beam()
+    b.pixel_coordinate_system = "healpix"
+    with pytest.raises(ValueError, match='pixel coordinate system must be "az_za"'):
+        grid_planes(b, 150e6, True)
+    b = _grid_beam()
+    b.axis2_array = b.axis2_array + 0.1
+    with pytest.raises(ValueError, match="must start at 0"):
+        grid_planes(b, 150e6, True)
+
+
+def test_grid_planes_frequency_interpolation():
+    az = np.radians(np.arange(8) * 45.0)
+    za = np.radians([0.0, 45.0, 90.0])
+    data = np.ones((2, 2, 2, 3, 8), complex)
+    data[:, :, 1] *= 3.0
+    b = GridBeam(data, az, za, freq_array=[100e6, 200e6])
+    planes, *_ = grid_planes(b, 150e6, True)
+    np.testing.assert_allclose(planes, 2.0)
+    with pytest.raises(ValueError, match="outside the beam's range"):
+        grid_planes(b, 250e6, True)
+
+
+def test_wrangle_beams_errors():
+    """tests/test_beams.py:58-88."""
+    g = GaussianBeam(sigma=0.2)
+    kw = dict(beam_idx=None, beam_list=[g], polarized=True, nant=3, freq=1e8)
+    with pytest.raises(ValueError, match="beam_idx must be provided"):
+        _wrangle_beams(**(kw | {"beam_list": [g, g]}))
+    with pytest.raises(ValueError, match="beam_idx must be length nant"):
+        _wrangle_beams(**(kw | {"beam_idx": np.zeros(2, int)}))
+    with pytest.raises(ValueError, match="beam_idx contains indices greater"):
+        _wrangle_beams(**(kw | {"beam_idx": np.array([0, 1, 1])}))
+    with pytest.raises(ValueError, match="beam type must be efield"):
+        _wrangle_beams(**(kw | {"beam_list": [BeamInterface(g, "power")]}))
+    bl, nb, bi = _wrangle_beams(**(kw | {"polarized": False}))
+    assert nb == 1 and bl[0].beam_type == "power"
+
+
+def test_gaussian_beam_sigma():
+    assert GaussianBeam(sigma=0.1).get_sigma(1e8) == 0.1
+    b = GaussianBeam(sigma=0.1, spectral_index=-1.0, reference_frequency=1e8)
+    assert math.isclose(b.get_sigma(2e8), 0.05)
+    with pytest.raises(ValueError):
+        GaussianBeam()
+
+
+def test_gmst_at_j2000_and_rotation_properties():
+    # GMST at 2000-01-01 12:00 UT1 is 18h41m50.54841s
+    g = astro.gmst_iau1982(astro.J2000)
+    assert abs(g - math.radians(280.46061837)) < 1e-9
+    # one sidereal day later GMST is back to the same value
+    g2 = astro.gmst_iau1982(astro.J2000 + 0.99726956633)
+    assert abs(math.remainder(g2 - g, 2 * math.pi)) < 1e-6
+    lat, lon = math.radians(-30.72), math.radians(21.43)
+    r = astro.eci_to_enu_rotation(2459863.3, lat, lon)
+    np.testing.assert_allclose(r @ r.T, np.eye(3), atol=1e-15)
+    assert abs(np.linalg.det(r) - 1) < 1e-14
+    # at J2000 the precession matrix is the identity
+    np.testing.assert_allclose(astro.precession_iau1976(astro.J2000), np.eye(3),
+                               atol=1e-15)
+    # the south celestial pole of date (J2000 vector: (N P)^T (0, 0, -1)) is
+    # due south at altitude -lat: ENU (0, cos lat ... ) with z = sin(-lat)
+    jd_tt = 2459863.3 + astro.TT_MINUS_UTC_S / 86400.0
+    dpsi, deps, eps0 = astro.nutation_iau1980_leading(jd_tt)
+    n_mat = astro._r1(-(eps0 + deps)) @ astro._r3(-dpsi) @ astro._r1(eps0)
+    pole = (n_mat @ astro.precession_iau1976(jd_tt)).T @ np.array([0, 0, -1.0])
+    enu = r @ pole
+    np.testing.assert_allclose(enu, [0.0, -math.cos(lat), math.sin(-lat)],
+                               atol=1e-12)
+    # the J2000 pole is off by the precession since J2000 (0.127 deg here)
+    off = math.degrees(math.acos(np.clip(pole @ [0, 0, -1.0], -1, 1)))
+    assert 0.12 < off < 0.14
+
+
+def test_duck_typed_inputs():
+    t = astro.Time([2459863.0, 2459863.1])
+    assert astro.times_jd(t).tolist() == [2459863.0, 2459863.1]
+    loc = astro.EarthLocation.from_geodetic(21.43, -30.72, 1000.0)
+    lat, lon = astro.location_lat_lon(loc)
+    assert math.isclose(lat, math.radians(-30.72))
+    sc = astro.SkyCoord(np.array([0.1, 0.2]), np.array([-0.3, 0.4]))
+    ra, dec = astro.skycoord_ra_dec(sc)
+    assert ra.tolist() == [0.1, 0.2] and len(sc) == 2
+
+
+def test_synthetic_inputs():
+    pos = synth.hex_array(350, 14.6, 0.02, seed=1)
+    assert pos.shape == (350, 3) and np.all(pos[:, 2] == 0)
+    d = np.hypot(*(pos[:, None, :2] - pos[None, :, :2]).transpose(2, 0, 1))
+    d[np.arange(350), np.arange(350)] = np.inf
+    assert abs(d.min(1).max() - 14.6) < 0.1
+    s = synth.powerlaw_fluxes(10000, 1)
+    assert s.min() >= 0.1 and s.max() <= 10.0
+    ra, dec = synth.healpix_centres(8)
+    assert ra.size == 768
+    v = np.stack([np.cos(ra) * np.cos(dec), np.sin(ra) * np.cos(dec), np.sin(dec)])
+    assert abs(v.sum(1)).max() < 1e-9  # centres balance on the sphere
+    assert np.sum(np.sin(dec) > 1e-12) == np.sum(np.sin(dec) < -1e-12)
+    ra, dec = synth.healpix_centres(1)
+    assert math.isclose(math.sin(dec[0]), 2 / 3) and math.isclose(ra[0], math.pi / 4)
+    beam = synth.harmonic_efield_beam(seed=3, naz=36, nza=10)
+    assert beam.data_array.shape == (2, 2, 1, 10, 36)
+    assert np.abs(beam.data_array[..., -1, :]).max() < 1e-6  # horizon taper
+
+
+def test_simulate_vis_requires_gpu_flag_and_validates():
+    import matvis_amd
+
+    with pytest.raises(NotImplementedError, match="use_gpu=True"):
+        matvis_amd.simulate_vis({0: (0, 0, 0)}, np.ones((1, 1)), np.zeros(1),
+                                np.zeros(1), np.array([1e8]),
+                                astro.Time([2459863.0]),
+                                [GaussianBeam(sigma=0.1)],
+                                synth.hera_location())
+    from matvis_amd.gpu.gpu import simulate_multi
+
+    for kw, msg in [({"source_buffer": 0.0}, "source_buffer must satisfy"),
+                    ({"memory_buffer": 1.1}, "memory_buffer must satisfy")]:
+        with pytest.raises(ValueError, match=msg):
+            simulate_multi(antpos=np.zeros((1, 3)), freqs=[1e8],
+                           times=astro.Time([2459863.0]),
+                           skycoords=astro.SkyCoord([0.0], [0.0]),
+                           telescope_loc=synth.hera_location(),
+                           fluxes=np.ones((1, 1)),
+                           beam_list=[GaussianBeam(sigma=0.1)], **kw)
+
+
+def test_matprod_antpairs_validation():
+    from matvis_amd.gpu.matprod import GPUMatMul
+
+    with pytest.raises(ValueError, match="out of range"):
+        GPUMatMul(1, 1, 3, np.array([[0, 3]]))
+    mp = GPUMatMul(2, 2, 3, None, precision=2)
+    assert mp.npairs == 9 and mp.antpairs[4].tolist() == [1, 1]
