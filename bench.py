@@ -138,11 +138,29 @@ def time_form(eng, steps, warmup, timer=None):
     return elapsed, per_step, nups
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def blas_threads() -> int:
+    """Threads numpy's BLAS runs with (the CPU baseline's core count)."""
+    try:
+        from threadpoolctl import threadpool_info
+
+        n = [i["num_threads"] for i in threadpool_info()
+             if i.get("user_api") == "blas"]
+        if n:
+            return int(max(n))
+    except Exception:
+        pass
+    return os.cpu_count() or 1
+
+
 def cpu_baseline(w, ninst_freqs=2):
-    """The oracle on one time x `ninst_freqs` frequencies, all host cores."""
+    """The oracle on one time x `ninst_freqs` frequencies, all BLAS threads."""
     import oracle as O
 
-    cores = os.cpu_count() or 1
+    cores = blas_threads()
     eci = O.point_source_crd_eq(w["ra"], w["dec"])
     from matvis_amd.beams import grid_planes
 
@@ -152,6 +170,7 @@ def cpu_baseline(w, ninst_freqs=2):
     t0 = time.perf_counter()
     outs = []
     for f in range(ninst_freqs):
+        log(f"cpu baseline instance {f + 1}/{ninst_freqs} ({cores} threads)")
         outs.append(O.simulate(antpos=w["antpos"], freq=w["freqs"][f], eci=eci,
                                rotations=w["rotations"][:1],
                                I_sky=w["fluxes"][:, f], beam=beam,
@@ -183,6 +202,7 @@ def main():
     from matvis_amd.coordinates import point_source_crd_eq
 
     torch.cuda.set_device(0)
+    log(f"building workload: {NANT} antennas x {args.nsrc} sources")
     w = build_workload(nsrc=args.nsrc)
     eci_h = point_source_crd_eq(w["ra"], w["dec"])
     forms = args.forms.split(",")
@@ -193,8 +213,11 @@ def main():
     for name in forms:
         form, nt, npd = ((FORM_SPLIT_BF16, 2, 4) if name == "bf16"
                          else (FORM_C128, 1, 1))
+        log(f"form {name}: setting up")
         eng = make_engine(w, form, nt, npd)
         eng.coords_eci_host = eci_h
+        log(f"form {name}: timing {args.steps} steps after {args.warmup} "
+            "warmup")
         timer = HerkTimer()
         steps = args.steps
         elapsed, per_step, nups = time_form(eng, steps, args.warmup, timer)
@@ -202,6 +225,8 @@ def main():
         k_mean = 2 * float(np.mean(nups))
         alg_flops = 8.0 * k_mean * m * (m + 1) / 2
         inst = steps * NFREQ
+        log(f"form {name}: {1e3 * elapsed / steps:.1f} ms/step, HERK "
+            f"{herk_ms:.3f} ms/launch")
         results[name] = dict(
             seconds=elapsed, ms_per_step=1e3 * elapsed / steps,
             vis_per_s=inst * vis_per_inst / elapsed,
