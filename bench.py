@@ -54,6 +54,23 @@ def build_workload(nsrc=NSRC, nant=NANT, nfreq=NFREQ, seed=1):
                 times=times, loc=loc, beam=beam, rotations=rot)
 
 
+FORMS = {
+    # name: (form, nterms, nprod)
+    "f16": ("FORM_SPLIT_F16", 2, 3),
+    "bf16": ("FORM_SPLIT_BF16", 2, 4),
+    "bf16x3": ("FORM_SPLIT_BF16", 3, 6),
+    "bf16-3p": ("FORM_SPLIT_BF16", 2, 3),
+    "fp64": ("FORM_C128", 1, 1),
+}
+FORM_LABEL = {
+    "f16": "split-FP16 (2 terms, 3 products)",
+    "bf16": "split-BF16 (2 terms, 4 products)",
+    "bf16x3": "split-BF16 (3 terms, 6 products)",
+    "bf16-3p": "split-BF16 (2 terms, 3 products)",
+    "fp64": "plain FP64",
+}
+
+
 def make_engine(w, form, nterms=1, nprod=1, precision=2):
     from matvis_amd import astro
     from matvis_amd.core.beams import _wrangle_beams
@@ -176,7 +193,8 @@ def cpu_baseline(w, ninst_freqs=2):
                                I_sky=w["fluxes"][:, f], beam=beam,
                                polarized=True))
     dt = time.perf_counter() - t0
-    vis = ninst_freqs * NANT * NANT * 4
+    nant = len(w["antpos"])
+    vis = ninst_freqs * nant * nant * 4
     return vis / dt, cores, outs
 
 
@@ -186,9 +204,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nsrc", type=int, default=NSRC)
+    ap.add_argument("--nant", type=int, default=NANT)
+    ap.add_argument("--nfreq", type=int, default=NFREQ)
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline (for profiling runs)")
-    ap.add_argument("--forms", default="bf16,fp64")
+    ap.add_argument("--forms", default="f16,bf16,fp64",
+                    help="comma list of " + ",".join(FORMS))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -198,21 +219,22 @@ def main():
 
     import torch
 
-    from matvis_amd._lib import FORM_C128, FORM_SPLIT_BF16
+    import matvis_amd._lib as L
     from matvis_amd.coordinates import point_source_crd_eq
 
     torch.cuda.set_device(0)
-    log(f"building workload: {NANT} antennas x {args.nsrc} sources")
-    w = build_workload(nsrc=args.nsrc)
+    nant, nfreq = args.nant, args.nfreq
+    log(f"building workload: {nant} antennas x {args.nsrc} sources")
+    w = build_workload(nsrc=args.nsrc, nant=nant, nfreq=nfreq)
     eci_h = point_source_crd_eq(w["ra"], w["dec"])
     forms = args.forms.split(",")
     results = {}
     outs = {}
-    vis_per_inst = NANT * NANT * 4
-    m = 2 * NANT
+    vis_per_inst = nant * nant * 4
+    m = 2 * nant
     for name in forms:
-        form, nt, npd = ((FORM_SPLIT_BF16, 2, 4) if name == "bf16"
-                         else (FORM_C128, 1, 1))
+        fname, nt, npd = FORMS[name]
+        form = getattr(L, fname)
         log(f"form {name}: setting up")
         eng = make_engine(w, form, nt, npd)
         eng.coords_eci_host = eci_h
@@ -224,13 +246,13 @@ def main():
         herk_ms = timer.mean_ms()
         k_mean = 2 * float(np.mean(nups))
         alg_flops = 8.0 * k_mean * m * (m + 1) / 2
-        inst = steps * NFREQ
+        inst = steps * nfreq
         log(f"form {name}: {1e3 * elapsed / steps:.1f} ms/step, HERK "
             f"{herk_ms:.3f} ms/launch")
         results[name] = dict(
             seconds=elapsed, ms_per_step=1e3 * elapsed / steps,
             vis_per_s=inst * vis_per_inst / elapsed,
-            src_bl_per_s=NFREQ * sum(n * NANT * (NANT + 1) / 2 for n in nups)
+            src_bl_per_s=nfreq * sum(n * nant * (nant + 1) / 2 for n in nups)
             / elapsed,
             p50_ms=float(np.percentile(per_step, 50)),
             p99_ms=float(np.percentile(per_step, 99)),
@@ -247,11 +269,13 @@ def main():
 
     def nerr(a, b):
         import oracle as O
-        return O.normalized_error(a, b, NANT, 2)
+        return O.normalized_error(a, b, nant, 2)
 
     err = {}
-    if "bf16" in outs and "fp64" in outs:
-        err["bf16_vs_fp64_gpu"] = nerr(outs["bf16"], outs["fp64"])
+    if "fp64" in outs:
+        for name in outs:
+            if name != "fp64":
+                err[f"{name}_vs_fp64_gpu"] = nerr(outs[name], outs["fp64"])
     cpu = None
     if not args.no_cpu:
         cpu_vis, cores, ref = cpu_baseline(w, 2)
@@ -260,13 +284,19 @@ def main():
             err[f"{name}_vs_oracle"] = nerr(outs[name], ref)
         cpu = {"value": cpu_vis, "unit": "vis/s", "cores": cores,
                "kind": "port",
-               "sample": "time 0 x 2 frequencies (100, 105.3 MHz) of the same "
-                         "workload: 2 instances, numpy/OpenBLAS ZGEMM over "
-                         "the padded source axis"}
+               "sample": "time 0 x first 2 frequencies of the same workload: "
+                         "2 instances through the numpy oracle (OpenBLAS "
+                         "ZGEMM over the padded source axis)"}
 
-    head = results.get("bf16") or results[forms[0]]
-    herk_bound = "mfma"
-    peak = MFMA_BF16_PEAK if "bf16" in results else FP64_PEAK
+    # headline: the fastest mixed-precision form within 1e-6 (normalised max)
+    # of the FP64 GPU path; FP64 itself if none is measured
+    ok = [n for n in results if n != "fp64"
+          and err.get(f"{n}_vs_fp64_gpu", (1.0,))[0] <= 1e-6]
+    head_name = (max(ok, key=lambda n: results[n]["vis_per_s"]) if ok
+                 else ("fp64" if "fp64" in results else forms[0]))
+    head = results[head_name]
+    is_split = head_name != "fp64"
+    peak = MFMA_BF16_PEAK if is_split else FP64_PEAK
     achieved = head["herk_tflops"]
     line = {
         "metric": "vis/s at 350 ant x 100k src pol, split-BF16 vs FP64 GPU "
@@ -280,24 +310,27 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16" if "bf16" in results else "f64",
-        "data": "synthetic (seeded hex array, random sky, harmonic E-field beam)",
-        "config": {"workload": "350 antennas x 100k sources, polarized, 20 "
-                               "freqs, bicubic c64 beam; step = 1 time x 20 "
-                               "freqs", "nant": NANT, "nsrc": args.nsrc,
-                   "nfreq": NFREQ, "form": "split-bf16 (2 terms, 4 products)"},
+        "dtype": {"f16": "f16", "fp64": "f64"}.get(head_name, "bf16"),
+        "data": "synthetic (seeded hex array, random sky, harmonic E-field "
+                "beam)",
+        "config": {"workload": f"{nant} antennas x {args.nsrc} sources, "
+                               f"polarized, {nfreq} freqs, bicubic c64 beam; "
+                               f"step = 1 time x {nfreq} freqs",
+                   "nant": nant, "nsrc": args.nsrc, "nfreq": nfreq,
+                   "form": FORM_LABEL[head_name]},
         "p50_ms": head["p50_ms"],
         "p99_ms": head["p99_ms"],
-        "roofline": {"bound": herk_bound, "achieved": achieved,
+        "roofline": {"bound": "mfma", "achieved": achieved,
                      "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / (peak / 1e12), "traffic": None},
         "cpu_baseline": cpu,
         "forms": results,
         "errors_normalised_max_median_elemmax_elemmedian": err,
     }
-    if "fp64" in results and "bf16" in results:
-        line["speedup_bf16_over_fp64"] = (results["bf16"]["vis_per_s"]
-                                          / results["fp64"]["vis_per_s"])
+    if "fp64" in results:
+        line["speedup_over_fp64"] = {
+            n: results[n]["vis_per_s"] / results["fp64"]["vis_per_s"]
+            for n in results if n != "fp64"}
     print(json.dumps(line))
 
 

@@ -155,8 +155,8 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(
     int naz, const double *__restrict__ grid, int is_power,
     const TC *__restrict__ crd, int64_t ld_crd, int64_t count,
     const int32_t *__restrict__ n_dev, cplx<T> *__restrict__ out,
-    int64_t ld_out, const double *__restrict__ az_in = nullptr,
-    const double *__restrict__ za_in = nullptr) {
+    int64_t ld_out, const double *__restrict__ az_in,
+    const double *__restrict__ za_in) {
   int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (p >= count) return;
   const int64_t n = n_dev ? min((int64_t)*n_dev, count) : count;
@@ -294,11 +294,13 @@ static void launch_interp(int order, dim3 g, hipStream_t s, const void *planes,
   if (order == 1)
     interp_kernel<T, TC, 1><<<g, kThreads, 0, s>>>(
         (const cplx<T> *)planes, nbeam, nfeed, nax, nza, naz, grid, is_power,
-        (const TC *)crd, ld_crd, count, n_dev, (cplx<T> *)out, ld_out);
+        (const TC *)crd, ld_crd, count, n_dev, (cplx<T> *)out, ld_out,
+        nullptr, nullptr);
   else
     interp_kernel<T, TC, 3><<<g, kThreads, 0, s>>>(
         (const cplx<T> *)planes, nbeam, nfeed, nax, nza, naz, grid, is_power,
-        (const TC *)crd, ld_crd, count, n_dev, (cplx<T> *)out, ld_out);
+        (const TC *)crd, ld_crd, count, n_dev, (cplx<T> *)out, ld_out,
+        nullptr, nullptr);
 }
 
 extern "C" int mv_beam_interp(const void *planes, int is_f64, int nbeam,

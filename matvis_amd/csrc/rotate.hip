@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kThreads) void cull_scatter_kernel(
       }
       if (keep) {
         int64_t dst = run + before + rank;
-        if (dst < nalloc) {
+        if (dst >= 0 && dst < nalloc) {
           crd[dst] = (T)rot_row(R, 0, x, y, z);
           crd[ld_crd + dst] = (T)rot_row(R, 1, x, y, z);
           crd[2 * ld_crd + dst] = (T)rot_row(R, 2, x, y, z);

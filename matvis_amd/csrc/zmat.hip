@@ -67,10 +67,14 @@ __global__ __launch_bounds__(kThreads) void zmat_parts_kernel(
   e.re *= f;
   e.im *= f;
   exptau[(int64_t)a * nsrc + s] = e;  // core/getz.py:80 updates exptau in place
-  int b = beam_idx ? beam_idx[a] : (nbeam == 1 ? 0 : a);
+  const int b = beam_idx ? beam_idx[a] : (nbeam == 1 ? 0 : a);
+  // an index outside the beam list is bad input: NaN, never an out-of-range read
+  const bool valid = (unsigned)b < (unsigned)nbeam;
+  const T nan = (T)NAN;
   for (int fd = 0; fd < nf; ++fd)
     for (int ax = 0; ax < na; ++ax) {
-      cplx<T> A = beam[((int64_t)(b * nf + fd) * na + ax) * nsrc + s];
+      cplx<T> A = valid ? beam[((int64_t)(b * nf + fd) * na + ax) * nsrc + s]
+                        : cplx<T>{nan, nan};
       z[((int64_t)a * nf + fd) * (na * nsrc) + (int64_t)ax * nsrc + s] =
           cmul(e, A);
     }
@@ -182,9 +186,13 @@ __global__ __launch_bounds__(kThreads) void zmat_fused_kernel(
   for (int a = a0; a < a1; ++a) {
     const int b = beam_idx ? beam_idx[a] : (nbeam == 1 ? 0 : a);
     if (b != cached) {
+      // an index outside the beam list is bad input: NaN, never an
+      // out-of-range read
+      const bool valid = (unsigned)b < (unsigned)nbeam;
 #pragma unroll
       for (int q = 0; q < nf * na; ++q)
-        A[q] = beam[((int64_t)b * nf * na + q) * ld_beam + s];
+        A[q] = valid ? beam[((int64_t)b * nf * na + q) * ld_beam + s]
+                     : cplx<T>{(T)NAN, (T)NAN};
       cached = b;
     }
     cplx<T> e = Phasor<T>::eval(reduced_turns(antpos + 3 * a, fc, sx, sy, sz));

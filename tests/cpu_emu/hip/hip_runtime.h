@@ -1,0 +1,175 @@
+// Host stand-in for <hip/hip_runtime.h>, used only to compile libmatvis_hip's
+// sources for the CPU so that every kernel's memory accesses can be checked
+// under AddressSanitizer.
+// each workgroup runs as fibers with real barrier semantics, and wave
+// intrinsics and MFMA are wave collectives computed from the operand layouts
+// the kernels assume (emu_runtime.cpp).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <type_traits>
+
+#define __global__
+#define __device__
+#define __host__
+#define __shared__ static
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+
+struct dim3 {
+  unsigned x, y, z;
+  dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct emu_idx_t {
+  unsigned x, y, z;
+};
+extern emu_idx_t threadIdx, blockIdx, blockDim, gridDim;
+
+typedef int hipError_t;
+constexpr hipError_t hipSuccess = 0;
+typedef struct emu_stream *hipStream_t;
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline const char *hipGetErrorString(hipError_t) { return "emulated"; }
+
+// SIMT runtime (emu_runtime.cpp)
+void emu_syncthreads();
+int emu_lane();
+void *emu_wave_slot(int lane);
+bool emu_lane_live(int lane);
+void emu_wave_barrier();
+
+inline void __syncthreads() { emu_syncthreads(); }
+inline unsigned long long __ballot(int p) {
+  *(int *)emu_wave_slot(emu_lane()) = p ? 1 : 0;
+  emu_wave_barrier();
+  unsigned long long m = 0;
+  for (int l = 0; l < 64; ++l)
+    if (emu_lane_live(l) && *(int *)emu_wave_slot(l)) m |= 1ull << l;
+  emu_wave_barrier();
+  return m;
+}
+template <class T> inline T __shfl_down(T v, unsigned off, int width = 64) {
+  static_assert(sizeof(T) <= 256, "slot size");
+  memcpy(emu_wave_slot(emu_lane()), &v, sizeof(T));
+  emu_wave_barrier();
+  const int l = emu_lane();
+  T r = v;
+  if ((unsigned)(l % width) + off < (unsigned)width)
+    memcpy(&r, emu_wave_slot(l + (int)off), sizeof(T));
+  emu_wave_barrier();
+  return r;
+}
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline unsigned __float_as_uint(float f) {
+  unsigned u;
+  memcpy(&u, &f, 4);
+  return u;
+}
+inline float __uint_as_float(unsigned u) {
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+inline void sincospi(double x, double *s, double *c) {
+  *s = sin(M_PI * x);
+  *c = cos(M_PI * x);
+}
+inline void sincospif(float x, float *s, float *c) {
+  *s = sinf((float)M_PI * x);
+  *c = cosf((float)M_PI * x);
+}
+using std::max;
+using std::min;
+
+// MFMA as wave collectives, with the operand layouts the kernels assume.
+// 16x16x4 (f64, f32): A[i][k] comes from lane i + 16 k, B[k][j] from lane
+// j + 16 k; lane l holds D rows 4 (l / 16) + r, column l % 16.
+template <class S, class C> inline C emu_mfma_16x16x4(S a, S b, C c) {
+  struct Op {
+    S a, b;
+  } me{a, b};
+  static_assert(sizeof(Op) <= 256, "slot size");
+  const int l = emu_lane();
+  memcpy(emu_wave_slot(l), &me, sizeof me);
+  emu_wave_barrier();
+  C d = c;
+  for (int r = 0; r < 4; ++r) {
+    const int i = 4 * (l / 16) + r, j = l % 16;
+    double dot = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      Op oa, ob;
+      memcpy(&oa, emu_wave_slot(i + 16 * k), sizeof oa);
+      memcpy(&ob, emu_wave_slot(j + 16 * k), sizeof ob);
+      dot += (double)oa.a * (double)ob.b;
+    }
+    d[r] = (std::remove_reference_t<decltype(c[r])>)((double)c[r] + dot);
+  }
+  emu_wave_barrier();
+  return d;
+}
+// 32x32x16 (bf16, f16; 8 per lane): A[i][k] is element k % 8 of lane
+// i + 32 (k / 8), B[k][j] element k % 8 of lane j + 32 (k / 8); lane l holds
+// D rows 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.
+template <class V, class C> inline C emu_mfma_32x32x16(V a, V b, C c) {
+  struct Op {
+    V a, b;
+  } me{a, b};
+  const int l = emu_lane();
+  memcpy(emu_wave_slot(l), &me, sizeof me);
+  emu_wave_barrier();
+  // the 16 products of one instruction are summed exactly and rounded once
+  // into the FP32 accumulator
+  C d = c;
+  for (int r = 0; r < 16; ++r) {
+    const int i = 8 * (r / 4) + 4 * (l / 32) + r % 4, j = l % 32;
+    double dot = 0.0;
+    for (int k = 0; k < 16; ++k) {
+      Op oa, ob;
+      memcpy(&oa, emu_wave_slot(i + 32 * (k / 8)), sizeof oa);
+      memcpy(&ob, emu_wave_slot(j + 32 * (k / 8)), sizeof ob);
+      dot += (double)(float)oa.a[k % 8] * (double)(float)ob.b[k % 8];
+    }
+    d[r] = (float)((double)c[r] + dot);
+  }
+  emu_wave_barrier();
+  return d;
+}
+#define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) \
+  emu_mfma_16x16x4(a, b, c)
+#define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, x, y, z) \
+  emu_mfma_16x16x4(a, b, c)
+#define __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, x, y, z) \
+  emu_mfma_32x32x16(a, b, c)
+#define __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, x, y, z) \
+  emu_mfma_32x32x16(a, b, c)
+
+struct emu_cfg_t {
+  dim3 grid, block;
+};
+inline emu_cfg_t emu_cfg(dim3 g, dim3 b, size_t = 0, hipStream_t = nullptr) {
+  return {g, b};
+}
+inline emu_cfg_t emu_cfg(dim3 g, dim3 b, size_t, void *) { return {g, b}; }
+
+void emu_run_block(const std::function<void()> &body);
+
+// Runs every block in turn; within a block, threads are fibers (see
+// emu_runtime.cpp).
+template <class... P, class... A>
+void emu_launch(void (*k)(P...), emu_cfg_t cfg, A... args) {
+  gridDim = {cfg.grid.x, cfg.grid.y, cfg.grid.z};
+  blockDim = {cfg.block.x, cfg.block.y, cfg.block.z};
+  const std::function<void()> body = [&]() { k(args...); };
+  for (unsigned bz = 0; bz < cfg.grid.z; ++bz)
+    for (unsigned by = 0; by < cfg.grid.y; ++by)
+      for (unsigned bx = 0; bx < cfg.grid.x; ++bx) {
+        blockIdx = {bx, by, bz};
+        emu_run_block(body);
+      }
+}
