@@ -173,6 +173,16 @@ def blas_threads() -> int:
     return os.cpu_count() or 1
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(w, ninst_freqs=2):
     """The oracle on one time x `ninst_freqs` frequencies, all BLAS threads."""
     import oracle as O
@@ -208,7 +218,7 @@ def main():
     ap.add_argument("--nfreq", type=int, default=NFREQ)
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline (for profiling runs)")
-    ap.add_argument("--forms", default="f16,bf16,fp64",
+    ap.add_argument("--forms", default="bf16,bf16x3,f16,fp64",
                     help="comma list of " + ",".join(FORMS))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -257,7 +267,10 @@ def main():
             p50_ms=float(np.percentile(per_step, 50)),
             p99_ms=float(np.percentile(per_step, 99)),
             herk_ms=herk_ms, herk_alg_flops=alg_flops,
-            herk_tflops=alg_flops / herk_ms / 1e9, n_up_mean=float(np.mean(nups)),
+            herk_exec_flops=npd * alg_flops,
+            herk_tflops=alg_flops / herk_ms / 1e9,
+            herk_exec_tflops=npd * alg_flops / herk_ms / 1e9,
+            n_up_mean=float(np.mean(nups)),
         )
         # keep one instance (time 0, first frequencies) for the error figures
         out = eng.new_output()
@@ -285,22 +298,30 @@ def main():
         cpu = {"value": cpu_vis, "unit": "vis/s", "cores": cores,
                "kind": "port",
                "sample": "time 0 x first 2 frequencies of the same workload: "
-                         "2 instances through the numpy oracle (OpenBLAS "
-                         "ZGEMM over the padded source axis)"}
+                         "2 instances through the numpy oracle (BLAS ZGEMM "
+                         "over the padded source axis) on " + cpu_model()}
 
-    # headline: the fastest mixed-precision form within 1e-6 (normalised max)
-    # of the FP64 GPU path; FP64 itself if none is measured
-    ok = [n for n in results if n != "fp64"
-          and err.get(f"{n}_vs_fp64_gpu", (1.0,))[0] <= 1e-6]
-    head_name = (max(ok, key=lambda n: results[n]["vis_per_s"]) if ok
-                 else ("fp64" if "fp64" in results else forms[0]))
+    # headline: the fastest split-BF16 form within 1e-6 (normalised max) of
+    # the FP64 GPU path; else the most accurate split-BF16 form measured;
+    # FP64 itself if no BF16 form ran
+    bf = [n for n in results if FORMS[n][0] == "FORM_SPLIT_BF16"]
+    ok = [n for n in bf if err.get(f"{n}_vs_fp64_gpu", (1.0,))[0] <= 1e-6]
+    if ok:
+        head_name = max(ok, key=lambda n: results[n]["vis_per_s"])
+    elif bf:
+        head_name = min(bf, key=lambda n: err.get(f"{n}_vs_fp64_gpu",
+                                                  (1.0,))[0])
+    else:
+        head_name = "fp64" if "fp64" in results else forms[0]
     head = results[head_name]
     is_split = head_name != "fp64"
     peak = MFMA_BF16_PEAK if is_split else FP64_PEAK
-    achieved = head["herk_tflops"]
+    # split forms: the executed term-product flops on the 16-bit MFMA
+    # (nprod x the useful 8 k m(m+1)/2); FP64: the useful flops
+    achieved = head["herk_exec_tflops"]
     line = {
-        "metric": "vis/s at 350 ant x 100k src pol, split-BF16 vs FP64 GPU "
-                  "@<=1e-6 rel err; p50/p99",
+        "metric": "vis/s at 350 ant\u00d7100k src pol, split-BF16 vs FP64 GPU "
+                  "@\u22641e-6 rel err; p50/p99",
         "value": head["vis_per_s"],
         "unit": "vis/s",
         "n_gpus": 1,
@@ -318,11 +339,15 @@ def main():
                                f"step = 1 time x {nfreq} freqs",
                    "nant": nant, "nsrc": args.nsrc, "nfreq": nfreq,
                    "form": FORM_LABEL[head_name]},
+        "meets_1e-6": bool(head_name in ok) if head_name != "fp64" else None,
         "p50_ms": head["p50_ms"],
         "p99_ms": head["p99_ms"],
         "roofline": {"bound": "mfma", "achieved": achieved,
                      "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / (peak / 1e12), "traffic": None},
+                     "frac": achieved / (peak / 1e12), "traffic": None,
+                     "kernel": "herk_split_kernel" if is_split
+                     else "herk_plain_kernel<double>",
+                     "useful_tflops": head["herk_tflops"]},
         "cpu_baseline": cpu,
         "forms": results,
         "errors_normalised_max_median_elemmax_elemmedian": err,

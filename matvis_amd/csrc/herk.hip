@@ -57,17 +57,21 @@ __device__ __forceinline__ KRange k_range(int64_t seg_len, int nseg,
 // ------------------------------------------------------------ plain forms
 
 template <typename T> struct PlainMfma;
+// Both take A[i][k] from lane i + 16 k and B[k][j] from lane j + 16 k; they
+// differ in where D lands: register r of lane l holds D[row(l, r)][l % 16].
 template <> struct PlainMfma<double> {
   typedef v4d acc;
   __device__ static acc op(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static int row(int g, int r) { return g + 4 * r; }
 };
 template <> struct PlainMfma<float> {
   typedef v4f acc;
   __device__ static acc op(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static int row(int g, int r) { return 4 * g + r; }
 };
 
 constexpr int kPT = 64;  // tile side, complex rows
@@ -178,7 +182,8 @@ __global__ __launch_bounds__(256) void herk_plain_kernel(
     __syncthreads();
   }
 
-  // 16x16 accumulator layout: lane l, register r -> row 4*(l/16) + r, col l%16
+  // 16x16 accumulator: lane l = 16 g + rl, register r -> row
+  // PlainMfma<T>::row(g, r), column rl
   cplx<T> *out = part + ((int64_t)slice * ntiles + tile) * kPT * kPT;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -186,7 +191,7 @@ __global__ __launch_bounds__(256) void herk_plain_kernel(
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = wr * 32 + i * 16 + 4 * g + r;
+        const int row = wr * 32 + i * 16 + PlainMfma<T>::row(g, r);
         const int col = wc * 32 + j * 16 + rl;
         out[row * kPT + col] = cplx<T>{acc_re[i][j][r], acc_im[i][j][r]};
       }

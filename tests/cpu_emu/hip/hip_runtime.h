@@ -87,9 +87,11 @@ inline void sincospif(float x, float *s, float *c) {
 using std::max;
 using std::min;
 
-// MFMA as wave collectives, with the operand layouts the kernels assume.
+// MFMA as wave collectives, with the operand layouts of gfx950 (checked on
+// the hardware with a one-wave probe).
 // 16x16x4 (f64, f32): A[i][k] comes from lane i + 16 k, B[k][j] from lane
-// j + 16 k; lane l holds D rows 4 (l / 16) + r, column l % 16.
+// j + 16 k; register r of lane l holds D column l % 16 and row
+// 4 (l / 16) + r for f32 but (l / 16) + 4 r for f64.
 template <class S, class C> inline C emu_mfma_16x16x4(S a, S b, C c) {
   struct Op {
     S a, b;
@@ -100,7 +102,8 @@ template <class S, class C> inline C emu_mfma_16x16x4(S a, S b, C c) {
   emu_wave_barrier();
   C d = c;
   for (int r = 0; r < 4; ++r) {
-    const int i = 4 * (l / 16) + r, j = l % 16;
+    const int i = sizeof(S) == 8 ? (l / 16) + 4 * r : 4 * (l / 16) + r;
+    const int j = l % 16;
     double dot = 0.0;
     for (int k = 0; k < 4; ++k) {
       Op oa, ob;

@@ -62,7 +62,7 @@ def patch_device_layer():
             pass
 
         def elapsed_time(self, other):
-            return 0.0
+            return 1.0  # no device clock here; a fixed nonzero stand-in
 
     class Stream:
         cuda_stream = 0
