@@ -9,9 +9,10 @@ Synthetic, seeded, generated here; inputs are resident in HBM before timing.
 
 A step is one time sample: rotation + cull once, then for each of the 20
 frequencies the beam, the fused phase/Z kernel, the HERK and the expansion
-into the API layout, results left in device memory.  Two forms are timed:
-split-BF16 (the headline `value`; Z = hi + lo bfloat16, four term products
-on BF16 MFMA with FP32 accumulation, FP64 cross-slice sums) and plain FP64.
+into the API layout, results left in device memory.  Forms timed: split-BF16
+(the headline `value`; Z = hi + lo bfloat16, three term products on BF16
+MFMA with FP32 accumulation per K-slice, FP64 cross-slice sums and an exact
+FP64 diagonal), its four-product variant, split-FP16 and plain FP64.
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build targets one GPU.
 """
@@ -56,17 +57,17 @@ def build_workload(nsrc=NSRC, nant=NANT, nfreq=NFREQ, seed=1):
 
 FORMS = {
     # name: (form, nterms, nprod)
-    "f16": ("FORM_SPLIT_F16", 2, 3),
-    "bf16": ("FORM_SPLIT_BF16", 2, 4),
+    "bf16": ("FORM_SPLIT_BF16", 2, 3),
+    "bf16x4": ("FORM_SPLIT_BF16", 2, 4),
     "bf16x3": ("FORM_SPLIT_BF16", 3, 6),
-    "bf16-3p": ("FORM_SPLIT_BF16", 2, 3),
+    "f16": ("FORM_SPLIT_F16", 2, 3),
     "fp64": ("FORM_C128", 1, 1),
 }
 FORM_LABEL = {
-    "f16": "split-FP16 (2 terms, 3 products)",
-    "bf16": "split-BF16 (2 terms, 4 products)",
+    "bf16": "split-BF16 (2 terms, 3 products)",
+    "bf16x4": "split-BF16 (2 terms, 4 products)",
     "bf16x3": "split-BF16 (3 terms, 6 products)",
-    "bf16-3p": "split-BF16 (2 terms, 3 products)",
+    "f16": "split-FP16 (2 terms, 3 products)",
     "fp64": "plain FP64",
 }
 
@@ -96,7 +97,8 @@ def make_engine(w, form, nterms=1, nprod=1, precision=2):
 
 
 class HerkTimer:
-    """HIP events around every HERK launch, on the launching stream."""
+    """HIP events around every HERK compute kernel, on the launching
+    stream (the split-K reduce that follows is not inside the pair)."""
 
     def __init__(self):
         import torch
@@ -173,6 +175,19 @@ def blas_threads() -> int:
     return os.cpu_count() or 1
 
 
+def measured_traffic(form: str, nant: int, nsrc: int):
+    """HBM bytes per HERK launch from the committed counter summary
+    (profiles/herk_traffic.json: FETCH_SIZE and WRITE_SIZE per dispatch,
+    each scaled by what the counter reads for a 1 GiB copy), when it covers
+    this form and workload; None otherwise."""
+    path = ROOT / "profiles" / "herk_traffic.json"
+    try:
+        rec = json.loads(path.read_text()).get(f"{form}:{nant}x{nsrc}")
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else float(rec["bytes_per_launch"])
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -189,9 +204,10 @@ def cpu_baseline(w, ninst_freqs=2):
 
     cores = blas_threads()
     eci = O.point_source_crd_eq(w["ra"], w["dec"])
-    from matvis_amd.beams import grid_planes
-
-    planes, daz, dza, azmin, _ = grid_planes(w["beam"], 150e6, True)
+    b = w["beam"]
+    planes, daz, dza, azmin = O.prepare_grid(
+        b.data_array, b.axis1_array, b.axis2_array, b.freq_array,
+        float(w["freqs"][0]), True, b.beam_type)
     beam = {"kind": "grid", "planes": planes[None], "daz": [daz],
             "dza": [dza], "azmin": [azmin], "order": 3}
     t0 = time.perf_counter()
@@ -218,8 +234,11 @@ def main():
     ap.add_argument("--nfreq", type=int, default=NFREQ)
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline (for profiling runs)")
-    ap.add_argument("--forms", default="bf16,bf16x3,f16,fp64",
+    ap.add_argument("--forms", default="bf16,bf16x4,f16,fp64",
                     help="comma list of " + ",".join(FORMS))
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (the "
+                         "known-size kernel that calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -233,6 +252,12 @@ def main():
     from matvis_amd.coordinates import point_source_crd_eq
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     nant, nfreq = args.nant, args.nfreq
     log(f"building workload: {nant} antennas x {args.nsrc} sources")
     w = build_workload(nsrc=args.nsrc, nant=nant, nfreq=nfreq)
@@ -344,7 +369,8 @@ def main():
         "p99_ms": head["p99_ms"],
         "roofline": {"bound": "mfma", "achieved": achieved,
                      "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / (peak / 1e12), "traffic": None,
+                     "frac": achieved / (peak / 1e12),
+                     "traffic": measured_traffic(head_name, nant, args.nsrc),
                      "kernel": "herk_split_kernel" if is_split
                      else "herk_plain_kernel<double>",
                      "useful_tflops": head["herk_tflops"]},

@@ -187,20 +187,46 @@ int mv_split_z(const void *z, int z_is_f64, int64_t rows, int nseg,
 /* Tile side (rows of V per workgroup tile) used by a HERK form. */
 int mv_herk_tile(int form);
 
-/* Workspace bytes for mv_herk with `nslices` K-slices at row count m. */
+/* Workspace bytes for mv_herk with room for `nslices` K-slices at row
+ * count m. */
 size_t mv_herk_workspace_bytes(int form, int m, int nslices);
+
+/* Slice capacity that covers every slice count mv_herk may pick for up to
+ * kmax columns (nseg * seg_len).  No reference counterpart: the reference's
+ * cuBLAS call has no split-K workspace. */
+int mv_herk_max_slices(int form, int m, int64_t kmax);
+
+/* The slice count mv_herk uses for seg_len columns per segment given a
+ * capacity of nslices (the kernels choose it on the device from the device
+ * column count by the same rule): enough slices that no FP32 accumulation
+ * chain exceeds the form's limit, then the one that best fills the GPU's
+ * resident workgroup slots.  Returns the count, or a negative status. */
+int mv_herk_slices(int form, int nterms, int nprod, int m, int nseg,
+                   int64_t seg_len, int nslices);
 
 /* Hermitian rank-k update over the upper triangle of tiles:
  * Vacc[r, c] (+)= scale * sum_k conj(Z[r, k]) Z[c, k]   for tile(r) <= tile(c)
  * Replaces GPUMatMul.compute -> zdotz -> cuBLAS zgemm/cgemm
  * (src/matvis/gpu/matprod.py:32-36, src/matvis/gpu/_cublas.py:12-65), which
- * computes the full m x m product.  K is split into `nslices` slices whose
- * partial tiles land in the workspace and are summed in FP64.
+ * computes the full m x m product.  K is split into slices (at most
+ * `nslices`, the workspace capacity; see mv_herk_slices) whose partial tiles
+ * land in the workspace and are summed in FP64 in a fixed order.
  * z0..z2: Z (plain) or its split terms (nterms, nprod significant products:
  * 3 = hi*hi + hi*lo + lo*hi, 4 adds lo*lo, 6 = the 3-term set up to order 2).
  * Z columns: nseg segments, each seg_len long (*seg_len_dev if non-NULL),
  * starting at seg*seg_stride; ldz is the row stride; both in complex units.
- * Vacc: complex128 (ldv x ldv, row-major), ldv >= ntile_side*tile.
+ * Split forms read whole tiles of 16 columns: every segment must have room
+ * for round_up(seg_len, 16) columns and hold zeros from its column count
+ * (*seg_len_dev when given) up to the next multiple of 16, as
+ * mv_zmat_fused leaves it.
+ * Split forms sum the diagonal V[r, r] exactly (FP64 over |t0 + t1 (+ t2)|^2)
+ * rather than from the MFMA accumulators: the MFMA adder keeps a window of
+ * about 2^-25 below its largest operand, so inside a long FP32 accumulation
+ * the small term products vanish and every diagonal element would sit low
+ * by their share (about 2^-18 for bf16); off the diagonal those products
+ * average out.
+ * Vacc: complex128 (ldv x ldv, row-major), ldv >= ntile_side*tile; holds
+ * V[r, c] for r <= c (other elements are scratch).
  * accumulate: 0 overwrites the upper tiles, 1 adds to them. */
 int mv_herk(int form, int nterms, int nprod, const void *z0, const void *z1,
             const void *z2, int m, int64_t ldz, int nseg, int64_t seg_stride,
@@ -208,13 +234,31 @@ int mv_herk(int form, int nterms, int nprod, const void *z0, const void *z1,
             double scale, double *vacc, int64_t ldv, int accumulate,
             void *workspace, size_t ws_bytes, mv_stream_t stream);
 
+/* mv_herk in its two launches, same arguments: the MFMA kernel that writes
+ * the slice partials to the workspace, and the FP64 reduce into Vacc.  A
+ * caller can record events between them (the compute kernel's own duration)
+ * or overlap other work.  No reference counterpart (one cuBLAS call there). */
+int mv_herk_compute(int form, int nterms, int nprod, const void *z0,
+                    const void *z1, const void *z2, int m, int64_t ldz,
+                    int nseg, int64_t seg_stride, int64_t seg_len,
+                    const int32_t *seg_len_dev, int nslices, double scale,
+                    double *vacc, int64_t ldv, int accumulate, void *workspace,
+                    size_t ws_bytes, mv_stream_t stream);
+int mv_herk_reduce(int form, int nterms, int nprod, const void *z0,
+                   const void *z1, const void *z2, int m, int64_t ldz,
+                   int nseg, int64_t seg_stride, int64_t seg_len,
+                   const int32_t *seg_len_dev, int nslices, double scale,
+                   double *vacc, int64_t ldv, int accumulate, void *workspace,
+                   size_t ws_bytes, mv_stream_t stream);
+
 /* Sum nsrc_v accumulators elementwise into dst (all ldv x ldv c128):
  * dst = sum_i src_i.  The chunk sum of MatProd.sum_chunks
  * (src/matvis/core/matprod.py:103-115, src/matvis/gpu/matprod.py:49-51). */
 int mv_vacc_sum(const double *src, int nsrc_v, int64_t stride, double *dst,
                 int64_t count, mv_stream_t stream);
 
-/* Expand the upper-tile accumulator into the API layout (npairs, nf, nf):
+/* Expand the accumulator (V[r, c] for r <= c, V[c, r] = conj(V[r, c])) into
+ * the API layout (npairs, nf, nf):
  * out[p, a, b] = V[i*nf + b, j*nf + a] for pair p = (i, j), the feed-swapped
  * layout of GPUMatMul.sum_chunks (src/matvis/gpu/matprod.py:38-63) and
  * CPUMatMul (src/matvis/cpu/matprod.py:24-30).  pairs: int32 (npairs, 2)

@@ -51,8 +51,14 @@ SIGNATURES: dict[str, list] = {
                    _L, _P],
     "mv_herk_tile": [_I],
     "mv_herk_workspace_bytes": [_I, _I, _I],
+    "mv_herk_max_slices": [_I, _I, _L],
+    "mv_herk_slices": [_I, _I, _I, _I, _I, _L, _I],
     "mv_herk": [_I, _I, _I, _P, _P, _P, _I, _L, _I, _L, _L, _P, _I, _D, _P,
                 _L, _I, _P, _S, _P],
+    "mv_herk_compute": [_I, _I, _I, _P, _P, _P, _I, _L, _I, _L, _L, _P, _I,
+                        _D, _P, _L, _I, _P, _S, _P],
+    "mv_herk_reduce": [_I, _I, _I, _P, _P, _P, _I, _L, _I, _L, _L, _P, _I,
+                       _D, _P, _L, _I, _P, _S, _P],
     "mv_vacc_sum": [_P, _I, _L, _P, _L, _P],
     "mv_vis_expand": [_P, _L, _I, _I, _I, _P, _L, _P, _I, _P],
 }

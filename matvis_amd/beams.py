@@ -149,9 +149,10 @@ def grid_planes(beam, freq: float, polarized: bool, beam_type=None):
     * unpolarized: an E-field beam becomes the xx power beam
       ``|E[0, x]|^2 + |E[1, x]|^2`` (core/beams.py:15-32); a power beam keeps
       its first polarization;
-    * azimuth wrap: a uniform grid covering [az0, az0 + 2 pi) without its end
-      point gets the first column appended at az0 + 2 pi, so sources in the
-      last cell interpolate across the seam;
+    * azimuth wrap, as pyuvdata's ``UVBeam._prepare_coordinate_data`` that
+      gpu/beams.py:15-19 calls: when span + spacing is within one spacing of
+      2 pi, 3 columns of wrapped data are added on each side and ``azmin``
+      moves 3 spacings below the first azimuth;
     * the map coordinates are [za / dza, (az - azmin) / daz]
       (gpu/beams.py:185), so the za grid must start at 0.
     """
@@ -183,11 +184,16 @@ def grid_planes(beam, freq: float, polarized: bool, beam_type=None):
             power = np.real(data[0, 0])
         planes = power[None, None].astype(np.float64)
         is_power = True
-    daz = float(az[1] - az[0])
-    dza = float(za[1] - za[0])
-    azmin = float(az.min())
-    uniform = np.allclose(np.diff(az), daz, rtol=1e-9, atol=1e-12)
-    span = az[-1] - az[0]
-    if uniform and abs(span + daz - 2 * np.pi) < 1e-6 * daz:
-        planes = np.concatenate([planes, planes[..., :1]], axis=-1)
-    return planes, daz, dza, azmin, is_power
+    step = az[1] - az[0]
+    span = abs(az[0] - az[-1]) + step
+    if np.isclose(span, 2 * np.pi, atol=step):
+        ext = 3
+        az = np.concatenate([az[0] + np.arange(-ext, 0) * step, az,
+                             az[-1] + np.arange(1, ext + 1) * step])
+        planes = np.concatenate([planes[..., -ext:], planes,
+                                 planes[..., :ext]], axis=-1)
+    # spacings as the reference takes them: first differences of the
+    # (padded) axes (gpu/beams.py:19)
+    daz = float(np.diff(az)[0])
+    dza = float(np.diff(za)[0])
+    return planes, daz, dza, float(az.min()), is_power

@@ -1,20 +1,37 @@
 // (d) V = conj(Z) Z^T as a Hermitian rank-k update on MFMA tiles.
 //
-// Only tiles (bi, bj) with bi <= bj of the upper triangle are computed.  K (the
-// source axis, times nax) is split into slices; each workgroup owns one
-// (tile, slice) pair and writes its partial tile to a workspace, and a reduce
-// kernel sums the slices in FP64 into the accumulator.  The order of that sum
-// is fixed, so results are deterministic.
+// Only tiles (bi, bj) with bi <= bj of the upper triangle are computed, and
+// inside them only the 64x64 (split) / 32x32 (plain) wave sub-tiles that hold
+// elements (r, c) with r <= c < m; the accumulator is defined on r <= c.
+//
+// K (the source axis, times nax) is cut into slices.  A work item is one
+// (tile, slice) pair; it writes its partial tile to the workspace, and a
+// reduce kernel sums the slices in FP64 in a fixed order, so results are
+// deterministic.  The number of active slices is chosen on the device from
+// the device-side column count (sources above the horizon are only known
+// there): enough slices that no FP32 accumulation chain exceeds `cmax` K
+// tiles, then the count among the next 64 that best fills the resident
+// workgroup slots.  The grid is at most one workgroup per slot; workgroups
+// loop over items, each XCD owning a contiguous run of slice-major items so
+// that workgroups sharing an L2 read the same K range of Z.
 //
 // Complex arithmetic on real MFMA: with X_r = (re_0, im_0, re_1, im_1, ...)
 // the row of Z read as reals, and Y_r = (im_0, -re_0, im_1, -re_1, ...),
 //   Re V[r, c] = X_r . X_c        Im V[r, c] = X_r . Y_c
 // so each output tile needs two real products over 2K reals: 8 flop per
 // complex multiply-add, with no extra passes over Z.
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "common.h"
 
 namespace mv {
 namespace {
+
+constexpr int kXcd = 8;
+
+__host__ __device__ inline int icdiv(int a, int b) { return (a + b - 1) / b; }
 
 // Upper-triangle tile index -> (bi, bj) with bi <= bj, row-major over bi.
 __device__ __forceinline__ void tile_coords(int t, int nb, int &bi, int &bj) {
@@ -27,31 +44,59 @@ __device__ __forceinline__ void tile_coords(int t, int nb, int &bi, int &bj) {
   bj = i + t;
 }
 
-// The hardware hands workgroup w to XCD w % 8.  Remap so that each XCD owns a
-// contiguous run of logical ids, ordered slice-major: workgroups that share an
-// XCD (and its L2) then work on neighbouring K-slices of every tile.
-__device__ __forceinline__ int xcd_remap(int w, int total) {
-  constexpr int nx = 8;
-  const int per = total / nx, rem = total % nx;
-  const int x = w % nx, j = w / nx;
-  return x < rem ? x * (per + 1) + j : rem * (per + 1) + (x - rem) * per + j;
-}
-
-struct KRange {
-  int64_t kt_per_seg, kt0, kt1;
+struct SliceCfg {
+  int max_slices;  // partial slots per tile in the workspace
+  int slots;       // workgroups resident at once (multiple of kXcd)
+  int cmax;        // at most this many K tiles per slice
+  int ovh;         // per-item cost beyond its K tiles, in K-tile units
 };
 
-// K tiles of width kc over nseg segments of seg_len columns, split evenly
-// across nslices; slice `slice` gets [kt0, kt1).
-__device__ __forceinline__ KRange k_range(int64_t seg_len, int nseg,
-                                          int nslices, int slice, int kc) {
-  KRange r;
-  r.kt_per_seg = cdiv(seg_len, kc);
-  const int64_t total = r.kt_per_seg * nseg;
-  const int64_t per = cdiv(total, nslices);
-  r.kt0 = min((int64_t)slice * per, total);
-  r.kt1 = min(r.kt0 + per, total);
-  return r;
+// Active slice count for `ktiles` K tiles over `ntiles` tiles.  Pure integer
+// function of its arguments: the compute and reduce kernels agree on it.
+__host__ __device__ inline int active_slices(int ktiles, int ntiles,
+                                             const SliceCfg &c) {
+  if (ktiles <= 0) return 1;
+  int lo = icdiv(ktiles, c.cmax);
+  lo = lo < 1 ? 1 : (lo > c.max_slices ? c.max_slices : lo);
+  int hi = lo + 64;
+  if (hi > c.max_slices) hi = c.max_slices;
+  if (hi > ktiles) hi = ktiles > lo ? ktiles : lo;
+  const int per_x = c.slots / kXcd > 0 ? c.slots / kXcd : 1;
+  int best = lo;
+  int64_t best_cost = INT64_MAX;
+  for (int ns = lo; ns <= hi; ++ns) {
+    const int rounds = icdiv(icdiv(ntiles * ns, kXcd), per_x);
+    const int64_t cost = (int64_t)rounds * (icdiv(ktiles, ns) + c.ovh);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = ns;
+    }
+  }
+  return best;
+}
+
+// Columns of each segment: the device count never exceeds the host bound the
+// buffers were sized for.
+__device__ __forceinline__ int64_t seg_columns(int64_t host,
+                                               const int32_t *dev) {
+  if (!dev) return host;
+  const int64_t d = *dev;
+  return d < 0 ? 0 : (d < host ? d : host);
+}
+
+// Item loop: workgroup b runs on XCD x = b % 8, which holds w_x =
+// ceil((g - x) / 8) of the g workgroups.  Each XCD owns a contiguous item
+// range proportional to w_x (empty when w_x = 0) and its workgroups sweep it.
+template <class F>
+__device__ __forceinline__ void for_items(int64_t items, F &&body) {
+  const int g = gridDim.x, b = blockIdx.x;
+  const int x = b % kXcd, j = b / kXcd;
+  int before = 0;
+  for (int y = 0; y < x; ++y) before += (g - y + kXcd - 1) / kXcd;
+  const int workers = (g - x + kXcd - 1) / kXcd;
+  const int64_t lo = items * before / g;
+  const int64_t hi = items * (before + workers) / g;
+  for (int64_t it = lo + j; it < hi; it += workers) body(it);
 }
 
 // ------------------------------------------------------------ plain forms
@@ -75,7 +120,7 @@ template <> struct PlainMfma<float> {
 };
 
 constexpr int kPT = 64;  // tile side, complex rows
-constexpr int kPK = 16;  // complex columns per LDS stage
+constexpr int kPK = 16;  // complex columns per LDS stage (one K tile)
 constexpr int kPRow = kPK + 1;  // LDS row length (one element of padding)
 
 // 256 threads = 4 waves in a 2x2 arrangement, each wave a 32x32 sub-tile made
@@ -85,122 +130,131 @@ template <typename T>
 __global__ __launch_bounds__(256) void herk_plain_kernel(
     const cplx<T> *__restrict__ z, int m, int64_t ldz, int nseg,
     int64_t seg_stride, int64_t seg_len_host,
-    const int32_t *__restrict__ seg_len_dev, int nb, int ntiles, int nslices,
-    cplx<T> *__restrict__ part) {
+    const int32_t *__restrict__ seg_len_dev, int nb, int ntiles,
+    SliceCfg cfg, cplx<T> *__restrict__ part) {
   typedef typename PlainMfma<T>::acc acc_t;
   __shared__ cplx<T> As[2][kPT * kPRow];
   __shared__ cplx<T> Bs[2][kPT * kPRow];
 
-  const int logical = xcd_remap(blockIdx.x, ntiles * nslices);
-  const int slice = logical / ntiles, tile = logical % ntiles;
-  int bi, bj;
-  tile_coords(tile, nb, bi, bj);
-  const bool diag = bi == bj;
-  // the device count never exceeds the host bound the buffers were sized for
-  const int64_t seg_len =
-      seg_len_dev ? min((int64_t)*seg_len_dev, seg_len_host) : seg_len_host;
-  const KRange kr = k_range(seg_len, nseg, nslices, slice, kPK);
+  const int64_t seg_len = seg_columns(seg_len_host, seg_len_dev);
+  const int64_t kt_per_seg = cdiv(seg_len, (int64_t)kPK);
+  const int ktiles = (int)(kt_per_seg * nseg);
+  const int ns = active_slices(ktiles, ntiles, cfg);
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int lrow = tid >> 2, lcol = (tid & 3) * 4;
-  const int64_t grow_a = (int64_t)bi * kPT + lrow;
-  const int64_t grow_b = (int64_t)bj * kPT + lrow;
-
-  acc_t acc_re[2][2], acc_im[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      acc_re[i][j] = acc_t{};
-      acc_im[i][j] = acc_t{};
-    }
-
-  cplx<T> ra[4], rb[4];
-  auto gload = [&](int64_t kt, int64_t grow, cplx<T> *r) {
-    const int64_t sg = kt / kr.kt_per_seg;
-    const int64_t c0 = (kt % kr.kt_per_seg) * kPK + lcol;
-    const cplx<T> *src = z + grow * ldz + sg * seg_stride + c0;
-    const bool row_ok = grow < m;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      r[q] = (row_ok && c0 + q < seg_len) ? src[q] : cplx<T>{(T)0, (T)0};
-  };
-  auto lstore = [&](cplx<T> *dst, const cplx<T> *r) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) dst[lrow * kPRow + lcol + q] = r[q];
-  };
-
-  if (kr.kt0 < kr.kt1) {
-    gload(kr.kt0, grow_a, ra);
-    lstore(As[0], ra);
-    if (!diag) {
-      gload(kr.kt0, grow_b, rb);
-      lstore(Bs[0], rb);
-    }
-  }
-  __syncthreads();
-
   const int g = lane >> 4, rl = lane & 15;
   const bool im_part = g & 1;
-  for (int64_t kt = kr.kt0; kt < kr.kt1; ++kt) {
-    const int buf = (int)((kt - kr.kt0) & 1);
-    const bool has_next = kt + 1 < kr.kt1;
-    if (has_next) {
-      gload(kt + 1, grow_a, ra);
-      if (!diag) gload(kt + 1, grow_b, rb);
-    }
-    const cplx<T> *Ab = As[buf];
-    const cplx<T> *Bb = diag ? As[buf] : Bs[buf];
+
+  for_items((int64_t)ntiles * ns, [&](int64_t it) {
+    const int slice = (int)(it / ntiles), tile = (int)(it % ntiles);
+    int bi, bj;
+    tile_coords(tile, nb, bi, bj);
+    const bool diag = bi == bj;
+    // sub-tiles below the diagonal or wholly in the row padding do no MFMA
+    const bool active = !(diag && wr > wc) && bi * kPT + wr * 32 < m &&
+                        bj * kPT + wc * 32 < m;
+    const int64_t kt0 = (int64_t)slice * ktiles / ns;
+    const int64_t kt1 = (int64_t)(slice + 1) * ktiles / ns;
+    const int64_t grow_a = (int64_t)bi * kPT + lrow;
+    const int64_t grow_b = (int64_t)bj * kPT + lrow;
+
+    acc_t acc_re[2][2], acc_im[2][2];
 #pragma unroll
-    for (int ks = 0; ks < kPK / 2; ++ks) {
-      const int col = 2 * ks + (g >> 1);
-      T a[2], bre[2], bim[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        cplx<T> v = Ab[(wr * 32 + i * 16 + rl) * kPRow + col];
-        a[i] = im_part ? v.im : v.re;
-      }
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        cplx<T> v = Bb[(wc * 32 + j * 16 + rl) * kPRow + col];
-        bre[j] = im_part ? v.im : v.re;
-        bim[j] = im_part ? -v.re : v.im;
+        acc_re[i][j] = acc_t{};
+        acc_im[i][j] = acc_t{};
       }
+
+    cplx<T> ra[4], rb[4];
+    auto gload = [&](int64_t kt, int64_t grow, cplx<T> *r) {
+      const int64_t sg = kt / kt_per_seg;
+      const int64_t c0 = (kt % kt_per_seg) * kPK + lcol;
+      const cplx<T> *src = z + grow * ldz + sg * seg_stride + c0;
+      const bool row_ok = grow < m;
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int q = 0; q < 4; ++q)
+        r[q] = (row_ok && c0 + q < seg_len) ? src[q] : cplx<T>{(T)0, (T)0};
+    };
+    auto lstore = [&](cplx<T> *dst, const cplx<T> *r) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          acc_re[i][j] = PlainMfma<T>::op(a[i], bre[j], acc_re[i][j]);
-          acc_im[i][j] = PlainMfma<T>::op(a[i], bim[j], acc_im[i][j]);
-        }
-    }
-    if (has_next) {
-      lstore(As[buf ^ 1], ra);
-      if (!diag) lstore(Bs[buf ^ 1], rb);
+      for (int q = 0; q < 4; ++q) dst[lrow * kPRow + lcol + q] = r[q];
+    };
+
+    if (kt0 < kt1) {
+      gload(kt0, grow_a, ra);
+      lstore(As[0], ra);
+      if (!diag) {
+        gload(kt0, grow_b, rb);
+        lstore(Bs[0], rb);
+      }
     }
     __syncthreads();
-  }
 
-  // 16x16 accumulator: lane l = 16 g + rl, register r -> row
-  // PlainMfma<T>::row(g, r), column rl
-  cplx<T> *out = part + ((int64_t)slice * ntiles + tile) * kPT * kPT;
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = wr * 32 + i * 16 + PlainMfma<T>::row(g, r);
-        const int col = wc * 32 + j * 16 + rl;
-        out[row * kPT + col] = cplx<T>{acc_re[i][j][r], acc_im[i][j][r]};
+    for (int64_t kt = kt0; kt < kt1; ++kt) {
+      const int buf = (int)((kt - kt0) & 1);
+      const bool has_next = kt + 1 < kt1;
+      if (has_next) {
+        gload(kt + 1, grow_a, ra);
+        if (!diag) gload(kt + 1, grow_b, rb);
       }
+      const cplx<T> *Ab = As[buf];
+      const cplx<T> *Bb = diag ? As[buf] : Bs[buf];
+      if (active) {
+#pragma unroll
+        for (int ks = 0; ks < kPK / 2; ++ks) {
+          const int col = 2 * ks + (g >> 1);
+          T a[2], bre[2], bim[2];
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            cplx<T> v = Ab[(wr * 32 + i * 16 + rl) * kPRow + col];
+            a[i] = im_part ? v.im : v.re;
+          }
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            cplx<T> v = Bb[(wc * 32 + j * 16 + rl) * kPRow + col];
+            bre[j] = im_part ? v.im : v.re;
+            bim[j] = im_part ? -v.re : v.im;
+          }
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              acc_re[i][j] = PlainMfma<T>::op(a[i], bre[j], acc_re[i][j]);
+              acc_im[i][j] = PlainMfma<T>::op(a[i], bim[j], acc_im[i][j]);
+            }
+        }
+      }
+      if (has_next) {
+        lstore(As[buf ^ 1], ra);
+        if (!diag) lstore(Bs[buf ^ 1], rb);
+      }
+      __syncthreads();
+    }
+
+    // 16x16 accumulator: lane l = 16 g + rl, register r -> row
+    // PlainMfma<T>::row(g, r), column rl
+    cplx<T> *out = part + ((int64_t)slice * ntiles + tile) * kPT * kPT;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = wr * 32 + i * 16 + PlainMfma<T>::row(g, r);
+          const int col = wc * 32 + j * 16 + rl;
+          out[row * kPT + col] = cplx<T>{acc_re[i][j][r], acc_im[i][j][r]};
+        }
+  });
 }
 
 // ------------------------------------------------------------ split forms
 
 constexpr int kST = 128;  // tile side, complex rows
-constexpr int kSK = 16;   // complex columns per LDS stage (2 MFMA k-steps)
+constexpr int kSK = 16;   // complex columns per LDS stage (one K tile)
 constexpr int kSRow = kSK + 4;  // LDS row stride in 32-bit words (80 bytes)
 
 template <bool F16> struct SplitMfma;
@@ -221,169 +275,312 @@ template <> struct SplitMfma<true> {
 __device__ __forceinline__ uint32_t swap_negate(uint32_t w) {
   return ((w >> 16) | (w << 16)) ^ 0x80000000u;
 }
+__device__ __forceinline__ u32x4 swap_negate4(u32x4 v) {
+  return u32x4{swap_negate(v[0]), swap_negate(v[1]), swap_negate(v[2]),
+               swap_negate(v[3])};
+}
 
 struct TermPtrs {
   const uint32_t *t[3];
 };
 
+typedef uint32_t StageBuf[kST * kSRow];
+
+// The two 16-bit halves of a packed (re, im) word as floats.
+template <bool F16> __device__ __forceinline__ float lo_half(uint32_t w) {
+  if constexpr (F16)
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+  else
+    return __builtin_bit_cast(float, w << 16);
+}
+template <bool F16> __device__ __forceinline__ float hi_half(uint32_t w) {
+  if constexpr (F16)
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+  else
+    return __builtin_bit_cast(float, w & 0xffff0000u);
+}
+
+// sum over the 8 complex values a thread stages of |t0 + t1 (+ t2)|^2, in
+// FP64 (two bf16 terms add exactly in FP32; three round once at 2^-24)
+template <int NT, bool F16>
+__device__ __forceinline__ double row_norm2(const u32x4 (*r)[2]) {
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    float re = 0.0f, im = 0.0f;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const uint32_t w = r[t][q >> 2][q & 3];
+      re += lo_half<F16>(w);
+      im += hi_half<F16>(w);
+    }
+    s += (double)re * re + (double)im * im;
+  }
+  return s;
+}
+
 // 256 threads = 4 waves in a 2x2 arrangement over a 128x128 tile; each wave
-// computes a 64x64 sub-tile as 2x2 blocks of 32x32x16 MFMA.  A lane's MFMA
-// operand is 8 consecutive 16-bit reals = 4 complex values of one Z row.
-// Term products (ta, tb) with ta + tb <= MAXORD are accumulated in FP32.
+// computes a 64x64 sub-tile as 2x2 blocks of 32x32x16 MFMA, whose K = 16
+// reals are 8 from lanes 0-31 and 8 from lanes 32-63.
+//
+// Term products are grouped so that small ones share an instruction with a
+// large one: a lane half h in {0, 1} supplies its term (a_h for A, b_h for B)
+// over the same 4 complex columns, so one MFMA adds t_a0 t_b0 + t_a1 t_b1
+// inside the instruction before the single FP32 rounding into the
+// accumulator.  A lo*lo product issued on its own falls below half an ulp of
+// a long accumulation and is rounded away; grouped with hi*hi it is kept.
+//   (2 terms, 3 products): hi*hi over 8 columns, then [hi|lo] x [lo|hi]
+//   (2 terms, 4 products): [hi|lo] x [hi|lo] and [hi|lo] x [lo|hi]
+//   (3 terms, 6 products): [h|m] x [h|m], [h|m] x [m|h], [h|l] x [l|h]
+// Each costs the same number of MFMAs as issuing the products separately.
+//
+// One K tile per LDS stage, single-buffered: the next tile's global loads are
+// in flight in registers while the current tile is consumed.  The column
+// panel is stored twice, as X and as Y = (im, -re), so the imaginary-part
+// MFMAs read their operand instead of forming it per use.  LDS per block:
+// 3 NT x 10 KB (two blocks per CU for two terms).
 template <int NT, int MAXORD, bool F16>
-__global__ __launch_bounds__(256) void herk_split_kernel(
+__global__ __launch_bounds__(256, 2) void herk_split_kernel(
     TermPtrs tp, int m, int64_t ldz, int nseg, int64_t seg_stride,
     int64_t seg_len_host, const int32_t *__restrict__ seg_len_dev, int nb,
-    int ntiles, int nslices, cplx<float> *__restrict__ part) {
-  __shared__ uint32_t As[2][NT][kST * kSRow];
-  __shared__ uint32_t Bs[2][NT][kST * kSRow];
+    int ntiles, SliceCfg cfg, cplx<float> *__restrict__ part,
+    double *__restrict__ dpart) {
+  __shared__ StageBuf Xa[NT];
+  __shared__ StageBuf Xb[NT];
+  __shared__ StageBuf Yb[NT];
 
-  const int logical = xcd_remap(blockIdx.x, ntiles * nslices);
-  const int slice = logical / ntiles, tile = logical % ntiles;
-  int bi, bj;
-  tile_coords(tile, nb, bi, bj);
-  const bool diag = bi == bj;
-  // the device count never exceeds the host bound the buffers were sized for
-  const int64_t seg_len =
-      seg_len_dev ? min((int64_t)*seg_len_dev, seg_len_host) : seg_len_host;
-  const KRange kr = k_range(seg_len, nseg, nslices, slice, kSK);
+  const int64_t seg_len = seg_columns(seg_len_host, seg_len_dev);
+  const int kps = (int)cdiv(seg_len, (int64_t)kSK);  // K tiles per segment
+  const int ktiles = kps * nseg;
+  const int ns = active_slices(ktiles, ntiles, cfg);
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   // global -> LDS: thread covers row tid/2, words [(tid%2)*8, +8)
   const int lrow = tid >> 1, lw = (tid & 1) * 8;
-  const int64_t grow_a = (int64_t)bi * kST + lrow;
-  const int64_t grow_b = (int64_t)bj * kST + lrow;
-
-  v16f acc_re[2][2], acc_im[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      acc_re[i][j] = v16f{};
-      acc_im[i][j] = v16f{};
-    }
-
-  u32x4 ra[NT][2], rb[NT][2];
-  auto gload = [&](int64_t kt, int64_t grow, u32x4 (*r)[2]) {
-    const int64_t sg = kt / kr.kt_per_seg;
-    const int64_t c0 = (kt % kr.kt_per_seg) * kSK + lw;
-    const bool row_ok = grow < m;
-    const bool full = row_ok && c0 + 8 <= seg_len;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const uint32_t *src = tp.t[t] + grow * ldz + sg * seg_stride + c0;
-      if (full) {
-        r[t][0] = *(const u32x4 *)src;
-        r[t][1] = *(const u32x4 *)(src + 4);
-      } else {
-        uint32_t w[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-          w[q] = (row_ok && c0 + q < seg_len) ? src[q] : 0u;
-        r[t][0] = u32x4{w[0], w[1], w[2], w[3]};
-        r[t][1] = u32x4{w[4], w[5], w[6], w[7]};
-      }
-    }
-  };
-  auto lstore = [&](uint32_t (*dst)[kST * kSRow], u32x4 (*r)[2]) {
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      *(u32x4 *)&dst[t][lrow * kSRow + lw] = r[t][0];
-      *(u32x4 *)&dst[t][lrow * kSRow + lw + 4] = r[t][1];
-    }
-  };
-
-  if (kr.kt0 < kr.kt1) {
-    gload(kr.kt0, grow_a, ra);
-    lstore(As[0], ra);
-    if (!diag) {
-      gload(kr.kt0, grow_b, rb);
-      lstore(Bs[0], rb);
-    }
-  }
-  __syncthreads();
-
   const int lr = lane & 31, lh = lane >> 5;
-  for (int64_t kt = kr.kt0; kt < kr.kt1; ++kt) {
-    const int buf = (int)((kt - kr.kt0) & 1);
-    const bool has_next = kt + 1 < kr.kt1;
-    if (has_next) {
-      gload(kt + 1, grow_a, ra);
-      if (!diag) gload(kt + 1, grow_b, rb);
-    }
-    uint32_t(*Ab)[kST * kSRow] = As[buf];
-    uint32_t(*Bb)[kST * kSRow] = diag ? As[buf] : Bs[buf];
-#pragma unroll
-    for (int ks = 0; ks < kSK / 8; ++ks) {
-      const int woff = ks * 8 + lh * 4;
-      u32x4 a[2][NT], b[2][NT], y[2][NT];
-#pragma unroll
-      for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          a[i][t] = *(const u32x4 *)&Ab[t][(wr * 64 + i * 32 + lr) * kSRow + woff];
-          b[i][t] = *(const u32x4 *)&Bb[t][(wc * 64 + i * 32 + lr) * kSRow + woff];
-          y[i][t] = u32x4{swap_negate(b[i][t][0]), swap_negate(b[i][t][1]),
-                          swap_negate(b[i][t][2]), swap_negate(b[i][t][3])};
-        }
-#pragma unroll
-      for (int ta = 0; ta < NT; ++ta)
-#pragma unroll
-        for (int tb = 0; tb < NT; ++tb) {
-          if (ta + tb > MAXORD) continue;
-#pragma unroll
-          for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-              acc_re[i][j] = SplitMfma<F16>::op(a[i][ta], b[j][tb], acc_re[i][j]);
-              acc_im[i][j] = SplitMfma<F16>::op(a[i][ta], y[j][tb], acc_im[i][j]);
-            }
-        }
-    }
-    if (has_next) {
-      lstore(As[buf ^ 1], ra);
-      if (!diag) lstore(Bs[buf ^ 1], rb);
-    }
-    __syncthreads();
-  }
 
-  // 32x32 accumulator layout: lane l, register r ->
-  //   row 8*(r/4) + 4*(l/32) + r%4, col l%32
-  cplx<float> *out = part + ((int64_t)slice * ntiles + tile) * kST * kST;
+  for_items((int64_t)ntiles * ns, [&](int64_t it) {
+    const int slice = (int)(it / ntiles), tile = (int)(it % ntiles);
+    int bi, bj;
+    tile_coords(tile, nb, bi, bj);
+    const bool diag = bi == bj;
+    const bool active = !(diag && wr > wc) && bi * kST + wr * 64 < m &&
+                        bj * kST + wc * 64 < m;
+    const int kt0 = (int)((int64_t)slice * ktiles / ns);
+    const int kt1 = (int)((int64_t)(slice + 1) * ktiles / ns);
+    const int grow_a = bi * kST + lrow, grow_b = bj * kST + lrow;
+    const bool ok_a = grow_a < m, ok_b = grow_b < m;
+    const int64_t roff_a = (int64_t)grow_a * ldz + lw;
+    const int64_t roff_b = (int64_t)grow_b * ldz + lw;
+    double dsum = 0.0;  // diagonal tiles: sum of |t0 + t1 (+ t2)|^2 of the row
+
+    v16f acc_re[2][2], acc_im[2][2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int row = wr * 64 + i * 32 + 8 * (r >> 2) + 4 * lh + (r & 3);
-        const int col = wc * 64 + j * 32 + lr;
-        out[row * kST + col] = cplx<float>{acc_re[i][j][r], acc_im[i][j][r]};
+      for (int j = 0; j < 2; ++j) {
+        acc_re[i][j] = v16f{};
+        acc_im[i][j] = v16f{};
       }
+
+    // position of the K tile to load next: segment sg, tile kc within it
+    int sg = kt0 / (kps > 0 ? kps : 1), kc = kt0 - sg * kps;
+    u32x4 ra[NT][2], rb[NT][2];
+    // whole K tiles: columns [seg_len, round_up(seg_len, 16)) of every
+    // segment are zero by contract, rows >= m are not read
+    auto gload1 = [&](int64_t roff, bool row_ok, u32x4 (*r)[2]) {
+      const int64_t off = roff + sg * seg_stride + kc * kSK;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const uint32_t *src = tp.t[t] + off;
+        if (row_ok) {
+          r[t][0] = *(const u32x4 *)src;
+          r[t][1] = *(const u32x4 *)(src + 4);
+        } else {
+          r[t][0] = u32x4{0u, 0u, 0u, 0u};
+          r[t][1] = u32x4{0u, 0u, 0u, 0u};
+        }
+      }
+    };
+    auto gload = [&]() {
+      gload1(roff_a, ok_a, ra);
+      if (!diag) gload1(roff_b, ok_b, rb);
+      if (++kc == kps) {
+        kc = 0;
+        ++sg;
+      }
+    };
+
+    if (kt0 < kt1) gload();
+    for (int kt = kt0; kt < kt1; ++kt) {
+      __syncthreads();  // every wave is done reading the previous tile
+      {
+        const int o = lrow * kSRow + lw;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          *(u32x4 *)&Xa[t][o] = ra[t][0];
+          *(u32x4 *)&Xa[t][o + 4] = ra[t][1];
+          if (diag) {
+            *(u32x4 *)&Yb[t][o] = swap_negate4(ra[t][0]);
+            *(u32x4 *)&Yb[t][o + 4] = swap_negate4(ra[t][1]);
+            if (t == NT - 1) dsum += row_norm2<NT, F16>(ra);
+          } else {
+            *(u32x4 *)&Xb[t][o] = rb[t][0];
+            *(u32x4 *)&Xb[t][o + 4] = rb[t][1];
+            *(u32x4 *)&Yb[t][o] = swap_negate4(rb[t][0]);
+            *(u32x4 *)&Yb[t][o + 4] = swap_negate4(rb[t][1]);
+          }
+        }
+      }
+      __syncthreads();
+      if (kt + 1 < kt1) gload();
+      if (!active) continue;
+      const StageBuf *Bx = diag ? Xa : Xb;
+      // operands of the 2x2 blocks: lane half lh reads term t_lh at word
+      // offset w_lh of its row
+      auto ld_a = [&](int t0, int t1, int w0, int w1, u32x4 (&a)[2]) {
+        const uint32_t *base = Xa[lh ? t1 : t0];
+        const int w = lh ? w1 : w0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+          a[i] = *(const u32x4 *)&base[(wr * 64 + i * 32 + lr) * kSRow + w];
+      };
+      auto ld_b = [&](int t0, int t1, int w0, int w1, u32x4 (&b)[2],
+                      u32x4 (&y)[2]) {
+        const int t = lh ? t1 : t0, w = lh ? w1 : w0;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int o = (wc * 64 + j * 32 + lr) * kSRow + w;
+          b[j] = *(const u32x4 *)&Bx[t][o];
+          y[j] = *(const u32x4 *)&Yb[t][o];
+        }
+      };
+      auto mma = [&](const u32x4 (&a)[2], const u32x4 (&b)[2],
+                     const u32x4 (&y)[2]) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            acc_re[i][j] = SplitMfma<F16>::op(a[i], b[j], acc_re[i][j]);
+            acc_im[i][j] = SplitMfma<F16>::op(a[i], y[j], acc_im[i][j]);
+          }
+      };
+      u32x4 a[2], b[2], y[2];
+      if constexpr (NT == 2 && MAXORD == 1) {
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+          ld_a(0, 0, 8 * ks, 8 * ks + 4, a);
+          ld_b(0, 0, 8 * ks, 8 * ks + 4, b, y);
+          mma(a, b, y);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          ld_a(0, 1, 4 * q, 4 * q, a);
+          ld_b(1, 0, 4 * q, 4 * q, b, y);
+          mma(a, b, y);
+        }
+      } else if constexpr (NT == 2) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          ld_a(0, 1, 4 * q, 4 * q, a);
+          ld_b(0, 1, 4 * q, 4 * q, b, y);
+          mma(a, b, y);
+          ld_b(1, 0, 4 * q, 4 * q, b, y);
+          mma(a, b, y);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          ld_a(0, 1, 4 * q, 4 * q, a);
+          ld_b(0, 1, 4 * q, 4 * q, b, y);
+          mma(a, b, y);
+          ld_b(1, 0, 4 * q, 4 * q, b, y);
+          mma(a, b, y);
+          ld_a(0, 2, 4 * q, 4 * q, a);
+          ld_b(2, 0, 4 * q, 4 * q, b, y);
+          mma(a, b, y);
+        }
+      }
+    }
+    __syncthreads();  // LDS is reused by the next item
+    if (diag) {
+      // two threads share each row; lanes tid and tid ^ 1 are in one wave
+      dsum += __shfl_xor(dsum, 1, 64);
+      if ((tid & 1) == 0)
+        dpart[((int64_t)slice * ntiles + tile) * kST + lrow] = dsum;
+    }
+
+    // 32x32 accumulator layout: lane l, register r ->
+    //   row 8*(r/4) + 4*(l/32) + r%4, col l%32.
+    // The thread's base offset is opaque to the optimizer so that the store
+    // addresses are formed here rather than kept live across the K loop.
+    cplx<float> *out = part + ((int64_t)slice * ntiles + tile) * kST * kST;
+    int obase = (wr * 64 + 4 * lh) * kST + wc * 64 + lr;
+    asm volatile("" : "+v"(obase));
+    out += obase;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int row = i * 32 + 8 * (r >> 2) + (r & 3);
+          out[row * kST + j * 32] =
+              cplx<float>{acc_re[i][j][r], acc_im[i][j][r]};
+        }
+  });
 }
 
 // ------------------------------------------------------------ epilogues
 
-// Sum the slice partials of every upper tile in FP64, scale, and store or add
-// into the accumulator.
+// Sum the active slice partials of every upper tile in FP64, scale, and store
+// or add into the accumulator.
+// With `dpart` (split forms) the diagonal elements of diagonal tiles take
+// the row norms summed there instead of the MFMA sums.
 template <typename TP>
 __global__ __launch_bounds__(256) void herk_reduce_kernel(
-    const cplx<TP> *__restrict__ part, int nslices, int ntiles, int nb,
-    int ts, double scale, cplx<double> *__restrict__ vacc, int64_t ldv,
+    const cplx<TP> *__restrict__ part, const double *__restrict__ dpart,
+    int ntiles, int nb, int ts, int64_t seg_len_host,
+    const int32_t *__restrict__ seg_len_dev, int nseg, int kc, SliceCfg cfg,
+    double scale, cplx<double> *__restrict__ vacc, int64_t ldv,
     int accumulate) {
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int tile = blockIdx.y;
   const int64_t tsz = (int64_t)ts * ts;
   if (e >= tsz) return;
-  double sr = 0.0, si = 0.0;
-  for (int s = 0; s < nslices; ++s) {
-    cplx<TP> v = part[((int64_t)s * ntiles + tile) * tsz + e];
-    sr += (double)v.re;
-    si += (double)v.im;
-  }
+  const int64_t seg_len = seg_columns(seg_len_host, seg_len_dev);
+  const int ktiles = (int)(cdiv(seg_len, (int64_t)kc) * nseg);
+  const int ns = active_slices(ktiles, ntiles, cfg);
   int bi, bj;
   tile_coords(tile, nb, bi, bj);
+  double sr = 0.0, si = 0.0;
+  if (dpart && bi == bj && e / ts == e % ts) {
+    for (int s = 0; s < ns; ++s)
+      sr += dpart[((int64_t)s * ntiles + tile) * ts + e / ts];
+  } else {
+    // four independent sums over slices s = 4q + j, combined in a fixed
+    // order: the loads of four slices are in flight at once
+    const cplx<TP> *src = part + (int64_t)tile * tsz + e;
+    const int64_t step = (int64_t)ntiles * tsz;
+    double r4[4] = {0.0, 0.0, 0.0, 0.0}, i4[4] = {0.0, 0.0, 0.0, 0.0};
+    int s = 0;
+    for (; s + 4 <= ns; s += 4) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const cplx<TP> v = src[(int64_t)(s + j) * step];
+        r4[j] += (double)v.re;
+        i4[j] += (double)v.im;
+      }
+    }
+    for (; s < ns; ++s) {
+      const cplx<TP> v = src[(int64_t)s * step];
+      r4[0] += (double)v.re;
+      i4[0] += (double)v.im;
+    }
+    sr = (r4[0] + r4[1]) + (r4[2] + r4[3]);
+    si = (i4[0] + i4[1]) + (i4[2] + i4[3]);
+  }
   const int64_t row = (int64_t)bi * ts + e / ts, col = (int64_t)bj * ts + e % ts;
   cplx<double> *dst = vacc + row * ldv + col;
   if (accumulate) {
@@ -404,10 +601,12 @@ __global__ __launch_bounds__(256) void vacc_sum_kernel(
   dst[e] = s;
 }
 
+// out[p, a, b] = V[i nf + b, j nf + a] from the r <= c half of the
+// accumulator, V[r, c] = conj(V[c, r]) for the other half.
 template <typename TO>
 __global__ __launch_bounds__(256) void expand_kernel(
-    const cplx<double> *__restrict__ vacc, int64_t ldv, int tile, int nant,
-    int nf, const int32_t *__restrict__ pairs, int64_t npairs,
+    const cplx<double> *__restrict__ vacc, int64_t ldv, int nant, int nf,
+    const int32_t *__restrict__ pairs, int64_t npairs,
     cplx<TO> *__restrict__ out) {
   const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int per = nf * nf;
@@ -428,7 +627,7 @@ __global__ __launch_bounds__(256) void expand_kernel(
   }
   const int64_t r = (int64_t)i * nf + b, c = (int64_t)j * nf + a;
   cplx<double> v;
-  if (r / tile <= c / tile) {
+  if (r <= c) {
     v = vacc[r * ldv + c];
   } else {
     v = vacc[c * ldv + r];
@@ -443,6 +642,48 @@ int split_maxord(int nterms, int nprod) {
   if (nterms == 3 && nprod == 6) return 2;
   return -1;
 }
+
+// Longest FP32 accumulation chain per slice, in K tiles of 16 columns.  The
+// split forms' diagonal is summed exactly, and off the diagonal the FP32
+// chain error stays near 1e-7 relative at 62 tiles and grows about as the
+// square root of the length; 192 tiles keeps it well under 1e-6 while the
+// partial tiles written per launch stay small.
+int form_cmax(int form) {
+  switch (form) {
+    case MV_FORM_C128: return 1 << 30;  // FP64 accumulation: no limit
+    case MV_FORM_C64: return 256;
+    default: return 192;
+  }
+}
+
+// Resident workgroups of `kernel` on the current device (CUs x blocks per
+// CU), rounded down to a multiple of the XCD count; cached per device.
+template <class K> int kernel_slots(K kernel) {
+  static std::mutex mu;
+  static std::map<std::pair<const void *, int>, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  const std::pair<const void *, int> key((const void *)kernel, dev);
+  std::lock_guard<std::mutex> lock(mu);
+  auto f = cache.find(key);
+  if (f != cache.end()) return f->second;
+  int cus = 0, per = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                            dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0) !=
+          hipSuccess)
+    return -1;
+  int slots = cus * (per > 0 ? per : 1);
+  slots = slots >= kXcd ? slots / kXcd * kXcd : kXcd;
+  cache[key] = slots;
+  return slots;
+}
+
+struct HerkPlan {
+  int ts, nb, ntiles, kc;
+  SliceCfg cfg;
+  int grid;
+};
 
 }  // namespace
 }  // namespace mv
@@ -459,21 +700,96 @@ extern "C" int mv_herk_tile(int form) {
   }
 }
 
+namespace {
+// split forms: FP32 partial tiles, then FP64 row norms of diagonal tiles
+size_t split_partial_bytes(int m, int nslices) {
+  const int64_t nb = cdiv((int64_t)m, (int64_t)kST);
+  return (size_t)nslices * (nb * (nb + 1) / 2) * kST * kST * 8;
+}
+}  // namespace
+
 extern "C" size_t mv_herk_workspace_bytes(int form, int m, int nslices) {
   const int ts = mv_herk_tile(form);
   if (ts == 0 || m <= 0 || nslices <= 0) return 0;
-  const int64_t nb = cdiv(m, ts);
+  const int64_t nb = cdiv((int64_t)m, (int64_t)ts);
   const int64_t ntiles = nb * (nb + 1) / 2;
+  if (ts == kST)
+    return split_partial_bytes(m, nslices) + (size_t)nslices * ntiles * kST * 8;
   const size_t elem = form == MV_FORM_C128 ? 16 : 8;
   return (size_t)nslices * ntiles * ts * ts * elem;
 }
 
-extern "C" int mv_herk(int form, int nterms, int nprod, const void *z0,
-                       const void *z1, const void *z2, int m, int64_t ldz,
-                       int nseg, int64_t seg_stride, int64_t seg_len,
-                       const int32_t *seg_len_dev, int nslices, double scale,
-                       double *vacc, int64_t ldv, int accumulate,
-                       void *workspace, size_t ws_bytes, mv_stream_t stream) {
+extern "C" int mv_herk_max_slices(int form, int m, int64_t kmax) {
+  const int ts = mv_herk_tile(form);
+  if (ts == 0 || m <= 0 || kmax < 0) return 0;
+  const int64_t ktiles = cdiv(kmax, (int64_t)16);
+  if (ktiles == 0) return 1;
+  const int64_t lo = cdiv(ktiles, (int64_t)form_cmax(form));
+  const int64_t ns = lo + 64 < ktiles ? lo + 64 : ktiles;
+  return (int)(ns < 1 ? 1 : ns);
+}
+
+namespace {
+// The kernel instantiation for (form, nterms, maxord) as a type-erased
+// pointer for occupancy queries.
+const void *herk_kernel(int form, int nterms, int maxord) {
+  switch (form) {
+    case MV_FORM_C128: return (const void *)&herk_plain_kernel<double>;
+    case MV_FORM_C64: return (const void *)&herk_plain_kernel<float>;
+    default: break;
+  }
+  const bool f16 = form == MV_FORM_SPLIT_F16;
+  if (nterms == 2 && maxord == 1)
+    return f16 ? (const void *)&herk_split_kernel<2, 1, true>
+               : (const void *)&herk_split_kernel<2, 1, false>;
+  if (nterms == 2)
+    return f16 ? (const void *)&herk_split_kernel<2, 2, true>
+               : (const void *)&herk_split_kernel<2, 2, false>;
+  return f16 ? (const void *)&herk_split_kernel<3, 2, true>
+             : (const void *)&herk_split_kernel<3, 2, false>;
+}
+
+int plan_herk(int form, int nterms, int maxord, int m, int nslices,
+              HerkPlan &p) {
+  p.ts = mv_herk_tile(form);
+  p.nb = (int)cdiv((int64_t)m, (int64_t)p.ts);
+  p.ntiles = p.nb * (p.nb + 1) / 2;
+  p.kc = p.ts == kPT ? kPK : kSK;
+  const int slots = kernel_slots(herk_kernel(form, nterms, maxord));
+  if (slots <= 0) return -1;
+  p.cfg = SliceCfg{nslices, slots, form_cmax(form),
+                   form == MV_FORM_C128 || form == MV_FORM_C64 ? 2 : 4};
+  const int64_t items = (int64_t)p.ntiles * nslices;
+  p.grid = (int)(items < slots ? items : slots);
+  return 0;
+}
+}  // namespace
+
+extern "C" int mv_herk_slices(int form, int nterms, int nprod, int m,
+                              int nseg, int64_t seg_len, int nslices) {
+  const int maxord = mv_herk_tile(form) == kST ? split_maxord(nterms, nprod)
+                                               : 0;
+  if (mv_herk_tile(form) == 0 || maxord < 0 || m < 1 || nseg < 1 ||
+      seg_len < 0 || nslices < 1)
+    return MV_ERR_ARG;
+  HerkPlan p;
+  if (plan_herk(form, nterms, maxord, m, nslices, p))
+    return MV_ERR_HIP;
+  const int ktiles = (int)(cdiv(seg_len, (int64_t)p.kc) * nseg);
+  return active_slices(ktiles, p.ntiles, p.cfg);
+}
+
+namespace {
+constexpr int kPhaseCompute = 1, kPhaseReduce = 2;
+
+// Validation shared by every phase, then the compute kernel (partial tiles
+// into the workspace) and/or the reduce kernel (workspace into Vacc).
+int herk_run(int phases, int form, int nterms, int nprod, const void *z0,
+             const void *z1, const void *z2, int m, int64_t ldz, int nseg,
+             int64_t seg_stride, int64_t seg_len, const int32_t *seg_len_dev,
+             int nslices, double scale, double *vacc, int64_t ldv,
+             int accumulate, void *workspace, size_t ws_bytes,
+             mv_stream_t stream) {
   const int ts = mv_herk_tile(form);
   MV_REQUIRE(ts > 0, "mv_herk: unknown form %d", form);
   MV_REQUIRE(z0 && vacc && workspace, "mv_herk: null pointer");
@@ -485,77 +801,127 @@ extern "C" int mv_herk(int form, int nterms, int nprod, const void *z0,
              (long long)seg_stride, (long long)seg_len);
   MV_REQUIRE(ldz >= (int64_t)(nseg - 1) * seg_stride + seg_len,
              "mv_herk: ldz %lld too small", (long long)ldz);
-  const int64_t nb = cdiv(m, ts);
+  MV_REQUIRE(cdiv(seg_len, (int64_t)16) * nseg < (int64_t)INT32_MAX / 2,
+             "mv_herk: K too large");
+  const int64_t nb = cdiv((int64_t)m, (int64_t)ts);
   const int64_t ntiles = nb * (nb + 1) / 2;
   MV_REQUIRE(ldv >= nb * ts, "mv_herk: ldv %lld < %lld (tile-padded m)",
              (long long)ldv, (long long)(nb * ts));
-  MV_REQUIRE(ntiles * nslices < (int64_t)INT32_MAX, "mv_herk: grid too large");
+  MV_REQUIRE(ntiles * nslices < (int64_t)INT32_MAX / 2,
+             "mv_herk: too many work items");
   MV_REQUIRE(ws_bytes >= mv_herk_workspace_bytes(form, m, nslices),
              "mv_herk: workspace too small (%zu < %zu)", ws_bytes,
              mv_herk_workspace_bytes(form, m, nslices));
-  hipStream_t st = as_stream(stream);
-  const unsigned grid = (unsigned)(ntiles * nslices);
-  if (form == MV_FORM_C128 || form == MV_FORM_C64) {
+  const bool plain = form == MV_FORM_C128 || form == MV_FORM_C64;
+  const int maxord = plain ? 0 : split_maxord(nterms, nprod);
+  if (plain)
     MV_REQUIRE(nterms == 1, "mv_herk: plain forms take nterms=1");
-    if (form == MV_FORM_C128) {
-      herk_plain_kernel<double><<<grid, 256, 0, st>>>(
-          (const cplx<double> *)z0, m, ldz, nseg, seg_stride, seg_len,
-          seg_len_dev, (int)nb, (int)ntiles, nslices,
-          (cplx<double> *)workspace);
-      if (int e = check_launch("mv_herk/plain64")) return e;
-      herk_reduce_kernel<double>
-          <<<dim3((unsigned)cdiv(ts * ts, 256), (unsigned)ntiles), 256, 0,
-             st>>>((const cplx<double> *)workspace, nslices, (int)ntiles,
-                   (int)nb, ts, scale, (cplx<double> *)vacc, ldv, accumulate);
-    } else {
-      herk_plain_kernel<float><<<grid, 256, 0, st>>>(
-          (const cplx<float> *)z0, m, ldz, nseg, seg_stride, seg_len,
-          seg_len_dev, (int)nb, (int)ntiles, nslices,
-          (cplx<float> *)workspace);
-      if (int e = check_launch("mv_herk/plain32")) return e;
-      herk_reduce_kernel<float>
-          <<<dim3((unsigned)cdiv(ts * ts, 256), (unsigned)ntiles), 256, 0,
-             st>>>((const cplx<float> *)workspace, nslices, (int)ntiles,
-                   (int)nb, ts, scale, (cplx<double> *)vacc, ldv, accumulate);
-    }
-    return check_launch("mv_herk/reduce");
-  }
-  const int maxord = split_maxord(nterms, nprod);
   MV_REQUIRE(maxord >= 0,
              "mv_herk: unsupported split (nterms=%d, nprod=%d); supported: "
              "(2,3) (2,4) (3,6)",
              nterms, nprod);
+  HerkPlan p;
+  if (plan_herk(form, nterms, maxord, m, nslices, p)) {
+    set_error(MV_ERR_HIP, "mv_herk: occupancy query failed");
+    return MV_ERR_HIP;
+  }
+  hipStream_t st = as_stream(stream);
+  const dim3 rgrid((unsigned)cdiv((int64_t)ts * ts, (int64_t)256),
+                   (unsigned)ntiles);
+  if (form == MV_FORM_C128) {
+    if (phases & kPhaseCompute) {
+      herk_plain_kernel<double><<<p.grid, 256, 0, st>>>(
+          (const cplx<double> *)z0, m, ldz, nseg, seg_stride, seg_len,
+          seg_len_dev, p.nb, p.ntiles, p.cfg, (cplx<double> *)workspace);
+      if (int e = check_launch("mv_herk/plain64")) return e;
+    }
+    if (!(phases & kPhaseReduce)) return MV_OK;
+    herk_reduce_kernel<double><<<rgrid, 256, 0, st>>>(
+        (const cplx<double> *)workspace, nullptr, p.ntiles, p.nb, ts, seg_len,
+        seg_len_dev, nseg, p.kc, p.cfg, scale, (cplx<double> *)vacc, ldv,
+        accumulate);
+    return check_launch("mv_herk/reduce");
+  }
+  if (form == MV_FORM_C64) {
+    if (phases & kPhaseCompute) {
+      herk_plain_kernel<float><<<p.grid, 256, 0, st>>>(
+          (const cplx<float> *)z0, m, ldz, nseg, seg_stride, seg_len,
+          seg_len_dev, p.nb, p.ntiles, p.cfg, (cplx<float> *)workspace);
+      if (int e = check_launch("mv_herk/plain32")) return e;
+    }
+    if (!(phases & kPhaseReduce)) return MV_OK;
+    herk_reduce_kernel<float><<<rgrid, 256, 0, st>>>(
+        (const cplx<float> *)workspace, nullptr, p.ntiles, p.nb, ts, seg_len,
+        seg_len_dev, nseg, p.kc, p.cfg, scale, (cplx<double> *)vacc, ldv,
+        accumulate);
+    return check_launch("mv_herk/reduce");
+  }
   MV_REQUIRE(z1 && (nterms < 3 || z2), "mv_herk: missing term arrays");
   MV_REQUIRE(ldz % 4 == 0 && (nseg == 1 || seg_stride % 4 == 0) &&
                  ((uintptr_t)z0 | (uintptr_t)z1 | (uintptr_t)(z2 ? z2 : z0)) %
                          16 == 0,
              "mv_herk: split terms need 16-byte aligned rows (ldz and "
              "seg_stride multiples of 4)");
+  const int64_t padded = cdiv(seg_len, (int64_t)kSK) * kSK;
+  MV_REQUIRE((nseg == 1 || seg_stride >= padded) &&
+                 ldz >= (int64_t)(nseg - 1) * seg_stride + padded,
+             "mv_herk: split terms need each segment padded to a multiple "
+             "of %d columns (seg_stride %lld, ldz %lld, seg_len %lld)",
+             kSK, (long long)seg_stride, (long long)ldz, (long long)seg_len);
   TermPtrs tp = {{(const uint32_t *)z0, (const uint32_t *)z1,
                   (const uint32_t *)z2}};
   const bool f16 = form == MV_FORM_SPLIT_F16;
 #define MV_SPLIT_LAUNCH(NT, MO, F)                                           \
-  herk_split_kernel<NT, MO, F><<<grid, 256, 0, st>>>(                        \
-      tp, m, ldz, nseg, seg_stride, seg_len, seg_len_dev, (int)nb,           \
-      (int)ntiles, nslices, (cplx<float> *)workspace)
-  if (nterms == 2 && maxord == 1) {
-    if (f16) MV_SPLIT_LAUNCH(2, 1, true);
-    else MV_SPLIT_LAUNCH(2, 1, false);
-  } else if (nterms == 2) {
-    if (f16) MV_SPLIT_LAUNCH(2, 2, true);
-    else MV_SPLIT_LAUNCH(2, 2, false);
-  } else {
-    if (f16) MV_SPLIT_LAUNCH(3, 2, true);
-    else MV_SPLIT_LAUNCH(3, 2, false);
+  herk_split_kernel<NT, MO, F><<<p.grid, 256, 0, st>>>(                      \
+      tp, m, ldz, nseg, seg_stride, seg_len, seg_len_dev, p.nb, p.ntiles,    \
+      p.cfg, (cplx<float> *)workspace, dpart)
+  double *dpart = (double *)((char *)workspace + split_partial_bytes(m, nslices));
+  if (phases & kPhaseCompute) {
+    if (nterms == 2 && maxord == 1) {
+      if (f16) MV_SPLIT_LAUNCH(2, 1, true);
+      else MV_SPLIT_LAUNCH(2, 1, false);
+    } else if (nterms == 2) {
+      if (f16) MV_SPLIT_LAUNCH(2, 2, true);
+      else MV_SPLIT_LAUNCH(2, 2, false);
+    } else {
+      if (f16) MV_SPLIT_LAUNCH(3, 2, true);
+      else MV_SPLIT_LAUNCH(3, 2, false);
+    }
+    if (int e = check_launch("mv_herk/split")) return e;
   }
 #undef MV_SPLIT_LAUNCH
-  if (int e = check_launch("mv_herk/split")) return e;
-  herk_reduce_kernel<float>
-      <<<dim3((unsigned)cdiv(ts * ts, 256), (unsigned)ntiles), 256, 0, st>>>(
-          (const cplx<float> *)workspace, nslices, (int)ntiles, (int)nb, ts,
-          scale, (cplx<double> *)vacc, ldv, accumulate);
+  if (!(phases & kPhaseReduce)) return MV_OK;
+  herk_reduce_kernel<float><<<rgrid, 256, 0, st>>>(
+      (const cplx<float> *)workspace, dpart, p.ntiles, p.nb, ts, seg_len,
+      seg_len_dev, nseg, p.kc, p.cfg, scale, (cplx<double> *)vacc, ldv,
+      accumulate);
   return check_launch("mv_herk/reduce");
 }
+
+}  // namespace
+
+#define MV_HERK_ARGS                                                        \
+  int form, int nterms, int nprod, const void *z0, const void *z1,          \
+      const void *z2, int m, int64_t ldz, int nseg, int64_t seg_stride,     \
+      int64_t seg_len, const int32_t *seg_len_dev, int nslices,             \
+      double scale, double *vacc, int64_t ldv, int accumulate,              \
+      void *workspace, size_t ws_bytes, mv_stream_t stream
+#define MV_HERK_PASS                                                        \
+  form, nterms, nprod, z0, z1, z2, m, ldz, nseg, seg_stride, seg_len,       \
+      seg_len_dev, nslices, scale, vacc, ldv, accumulate, workspace,        \
+      ws_bytes, stream
+
+extern "C" int mv_herk(MV_HERK_ARGS) {
+  return herk_run(kPhaseCompute | kPhaseReduce, MV_HERK_PASS);
+}
+extern "C" int mv_herk_compute(MV_HERK_ARGS) {
+  return herk_run(kPhaseCompute, MV_HERK_PASS);
+}
+extern "C" int mv_herk_reduce(MV_HERK_ARGS) {
+  return herk_run(kPhaseReduce, MV_HERK_PASS);
+}
+#undef MV_HERK_ARGS
+#undef MV_HERK_PASS
 
 extern "C" int mv_vacc_sum(const double *src, int nsrc_v, int64_t stride,
                            double *dst, int64_t count, mv_stream_t stream) {
@@ -563,8 +929,8 @@ extern "C" int mv_vacc_sum(const double *src, int nsrc_v, int64_t stride,
   MV_REQUIRE(nsrc_v >= 1 && count >= 0 && stride >= count,
              "mv_vacc_sum: bad sizes");
   if (count == 0) return MV_OK;
-  vacc_sum_kernel<<<(unsigned)cdiv(count, 256), 256, 0, as_stream(stream)>>>(
-      src, nsrc_v, stride, dst, count);
+  vacc_sum_kernel<<<(unsigned)cdiv(count, (int64_t)256), 256, 0,
+                    as_stream(stream)>>>(src, nsrc_v, stride, dst, count);
   return check_launch("mv_vacc_sum");
 }
 
@@ -578,15 +944,15 @@ extern "C" int mv_vis_expand(const double *vacc, int64_t ldv, int tile,
              "mv_vis_expand: bad sizes");
   const int64_t total = npairs * nfeed * nfeed;
   if (total == 0) return MV_OK;
-  dim3 g((unsigned)cdiv(total, 256));
+  dim3 g((unsigned)cdiv(total, (int64_t)256));
   hipStream_t st = as_stream(stream);
   if (out_is_f64)
     expand_kernel<double><<<g, 256, 0, st>>>((const cplx<double> *)vacc, ldv,
-                                             tile, nant, nfeed, pairs, npairs,
+                                             nant, nfeed, pairs, npairs,
                                              (cplx<double> *)out);
   else
     expand_kernel<float><<<g, 256, 0, st>>>((const cplx<double> *)vacc, ldv,
-                                            tile, nant, nfeed, pairs, npairs,
+                                            nant, nfeed, pairs, npairs,
                                             (cplx<float> *)out);
   return check_launch("mv_vis_expand");
 }

@@ -74,7 +74,8 @@ class VisibilityEngine:
             self.npairs = len(self.pairs_h)
         self._ready = False
         #: optional (before, after) callables taking the stream, run around
-        #: every HERK launch (event timing of the dominant kernel)
+        #: every HERK compute kernel (event timing of the dominant kernel;
+        #: the FP64 reduce that follows it is outside the pair)
         self.herk_hooks = None
 
     # ---------------------------------------------------------------- setup
@@ -204,9 +205,9 @@ class VisibilityEngine:
                               seg_stride=self.seg, seg_len=self.nsrc_alloc,
                               seg_len_dev=n_dev, vacc=self.vacc,
                               accumulate=c > 0, scale=1.0 / (s * s),
-                              nslices=self.nslices, stream=stream)
-                if self.herk_hooks:
-                    self.herk_hooks[1](stream)
+                              nslices=self.nslices, stream=stream,
+                              between=(self.herk_hooks[1] if self.herk_hooks
+                                       else None))
             expand(self.vacc, self.herk.ldv, self.herk.tile, self.nant,
                    self.nfeed, self.pairs_dev, self.npairs, out_dev[fi],
                    stream)

@@ -1,7 +1,9 @@
 """Host driver of the Hermitian rank-k update kernels (``mv_herk``).
 
-A :class:`Herk` fixes the numeric form and the row count m = nfeed * nant,
-owns the split-K workspace, and picks the number of K-slices.
+A :class:`Herk` fixes the numeric form and the row count m = nfeed * nant
+and owns the split-K workspace.  The kernels choose the number of active
+K-slices on the device (the column count after the horizon cull is only known
+there); the host sizes the workspace for the largest count they may choose.
 """
 
 from __future__ import annotations
@@ -36,8 +38,7 @@ def is_split(form: int) -> bool:
 class Herk:
     """V (+)= scale * conj(Z) Z^T over the upper tiles, into an FP64 buffer."""
 
-    def __init__(self, form: int, m: int, nterms: int = 1, nprod: int = 1,
-                 target_workgroups: int = 1024, min_ktiles: int = 4):
+    def __init__(self, form: int, m: int, nterms: int = 1, nprod: int = 1):
         if is_split(form):
             if (nterms, nprod) not in SPLIT_VARIANTS:
                 raise ValueError(
@@ -51,16 +52,22 @@ class Herk:
         self.nb = cdiv(m, self.tile)
         self.ntiles = self.nb * (self.nb + 1) // 2
         self.ldv = self.nb * self.tile
-        self.target_workgroups = target_workgroups
-        self.min_ktiles = min_ktiles
         self._ws = None
 
     def nslices(self, kmax: int) -> int:
-        """K-slices for at most kmax complex columns: enough workgroups to
-        fill the chip, each with at least `min_ktiles` K tiles."""
-        ktiles = max(1, cdiv(kmax, K_TILE))
-        ns = cdiv(self.target_workgroups, self.ntiles)
-        return max(1, min(ns, ktiles // self.min_ktiles))
+        """Slice capacity covering every count the kernels may choose for up
+        to kmax complex columns."""
+        return max(1, int(lib().mv_herk_max_slices(self.form, self.m,
+                                                   int(kmax))))
+
+    def active_slices(self, nseg: int, seg_len: int, nslices: int) -> int:
+        """The slice count the kernels use for seg_len columns per segment."""
+        n = lib().mv_herk_slices(self.form, self.nterms, self.nprod, self.m,
+                                 nseg, int(seg_len), nslices)
+        if n < 1:
+            call("mv_herk_slices", self.form, self.nterms, self.nprod, self.m,
+                 nseg, int(seg_len), nslices)
+        return int(n)
 
     def workspace(self, nslices: int):
         need = lib().mv_herk_workspace_bytes(self.form, self.m, nslices)
@@ -73,16 +80,23 @@ class Herk:
 
     def run(self, z_terms, ldz: int, nseg: int, seg_stride: int, seg_len: int,
             vacc, accumulate: bool, seg_len_dev=None, scale: float = 1.0,
-            nslices: int | None = None, stream=None):
+            nslices: int | None = None, stream=None, between=None):
+        """Enqueue the HERK; ``between(stream)``, if given, runs after the
+        compute kernel is enqueued and before the reduce."""
         kmax = nseg * seg_len
         ns = nslices or self.nslices(kmax)
         ws = self.workspace(ns)
         zp = [D.ptr(t) for t in z_terms] + [None] * (3 - len(z_terms))
-        call("mv_herk", self.form, self.nterms, self.nprod, zp[0], zp[1],
-             zp[2], self.m, ldz, nseg, seg_stride, seg_len,
-             D.ptr(seg_len_dev), ns, float(scale), D.ptr(vacc), self.ldv,
-             int(bool(accumulate)), D.ptr(ws), ws.numel(),
-             D.stream_ptr(stream))
+        args = (self.form, self.nterms, self.nprod, zp[0], zp[1], zp[2],
+                self.m, ldz, nseg, seg_stride, seg_len, D.ptr(seg_len_dev), ns,
+                float(scale), D.ptr(vacc), self.ldv, int(bool(accumulate)),
+                D.ptr(ws), ws.numel(), D.stream_ptr(stream))
+        if between is None:
+            call("mv_herk", *args)
+        else:
+            call("mv_herk_compute", *args)
+            between(stream)
+            call("mv_herk_reduce", *args)
         return vacc
 
 

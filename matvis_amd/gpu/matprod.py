@@ -108,11 +108,12 @@ GPUVectorLoop = GPUVectorDot
 
 class GPUMatMulSplitBF16(GPUMatMul):
     """Mixed precision: Z = hi + lo in bfloat16, products hi*hi + hi*lo +
-    lo*hi + lo*lo (``split_products=3`` drops lo*lo) on BF16 MFMA."""
+    lo*hi on BF16 MFMA (``split_products=4`` adds lo*lo), FP32 accumulation
+    per K-slice, FP64 across slices, the diagonal summed exactly."""
 
     split_form = FORM_SPLIT_BF16
     default_terms = 2
-    default_products = 4
+    default_products = 3
 
     def _scale(self, z) -> float:
         return 1.0
@@ -120,10 +121,14 @@ class GPUMatMulSplitBF16(GPUMatMul):
     def compute(self, z, out):
         z = self._check_z(z)
         k = z.shape[1]
-        ldt = -(-k // 4) * 4
+        # the HERK reads whole 16-column tiles: columns [k, ldt) must be zero
+        ldt = max(16, -(-k // 16) * 16)
         if getattr(self, "_terms", None) is None or self._terms[0].shape[1] < ldt:
             self._terms = [D.zeros((self.m, ldt), np.int32)
                            for _ in range(self.nterms)]
+        else:
+            for term in self._terms:
+                term[:, k:ldt].zero_()
         t = self._terms + [None] * (3 - self.nterms)
         s = self._scale(z)
         call("mv_split_z", D.ptr(z), int(z.dtype == torch.complex128), self.m,

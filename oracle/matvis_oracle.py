@@ -29,6 +29,11 @@ Parity status per function (see ``tests/test_oracle.py``):
 ``rotate``                  pinned: brute force vs an explicit matrix product
 ``select_chunk``            pinned: ascending-order compaction, zero padding
 ``enu_to_az_za``            pinned: E/N/U conventions (test_coordinates.py:108-160)
+``prepare_grid``            **parity unpinned** for its two pyuvdata steps
+                            (frequency interpolation, the 3-column azimuth
+                            padding of _prepare_coordinate_data); pinned by
+                            invariants only: grid nodes are reproduced, padded
+                            columns equal the wrapped data
 ``interp_grid_beam``        pinned: scipy map_coordinates; node identity and
                             linear exactness (test_beam_interp_gpu.py:12-64)
 ``gaussian_beam``           **parity unpinned**: restates pyuvdata's
@@ -153,6 +158,55 @@ def enu_to_az_za(enu_e, enu_n, orientation="astropy", periodic_azimuth=True):
     if periodic_azimuth:
         az = np.mod(az, 2 * np.pi)
     return az, za
+
+
+def prepare_grid(data_array, az, za, freq_array, freq: float,
+                 polarized: bool, beam_type: str = "efield"):
+    """Interpolation grid of one UVBeam-shaped az/za beam at one frequency.
+
+    The preparation the reference applies before ``gpu_beam_interpolation``:
+
+    * frequency: ``UVBeam.interp(freq_array=[freq])`` (core/beams.py:82-93),
+      restated as linear interpolation between the bracketing frequencies;
+    * unpolarized: ``prepare_beam_unpolarized`` (core/beams.py:15-32): an
+      E-field beam becomes the xx power beam ``sum_vec |E[vec, x]|^2`` (the
+      first feed), a power beam keeps its first polarization;
+    * ``prepare_for_map_coords`` (gpu/beams.py:15-19), which takes pyuvdata's
+      ``UVBeam._prepare_coordinate_data``: when the azimuth axis covers the
+      circle (span + spacing within one spacing of 2 pi) the grid is padded
+      with 3 columns of wrapped data on each side; ``daz`` is the first
+      spacing, ``azmin`` the padded axis's minimum.
+
+    Returns ``(planes (nax, nfeed, nza, naz'), daz, dza, azmin)``.
+    PARITY UNPINNED for the frequency interpolation and the padding rule
+    (pyuvdata 3.2.4 algorithms, absent here); pinned by invariants only
+    (nodes reproduce the grid; a periodic beam agrees at az and az + 2 pi).
+    """
+    data = np.asarray(data_array)
+    az = np.asarray(az, dtype=np.float64)
+    za = np.asarray(za, dtype=np.float64)
+    fa = np.atleast_1d(np.asarray(freq_array, dtype=np.float64))
+    if fa.size == 1:
+        d = data[:, :, 0]
+    else:
+        i = int(np.clip(np.searchsorted(fa, freq) - 1, 0, fa.size - 2))
+        w = (freq - fa[i]) / (fa[i + 1] - fa[i])
+        d = (1 - w) * data[:, :, i] + w * data[:, :, i + 1]
+    if polarized:
+        planes = d[:2, :2].astype(np.complex128)
+    elif beam_type == "efield":
+        planes = (np.abs(d[:, 0]) ** 2).sum(axis=0)[None, None]
+    else:
+        planes = np.real(d[0, 0])[None, None].astype(np.float64)
+    step = az[1] - az[0]
+    span = abs(az[0] - az[-1]) + step
+    if np.isclose(span, 2 * np.pi, atol=step):
+        ext = 3
+        az = np.concatenate([az[0] + np.arange(-ext, 0) * step, az,
+                             az[-1] + np.arange(1, ext + 1) * step])
+        planes = np.concatenate([planes[..., -ext:], planes,
+                                 planes[..., :ext]], axis=-1)
+    return planes, float(np.diff(az)[0]), float(np.diff(za)[0]), float(az.min())
 
 
 def interp_grid_beam(planes, daz, dza, azmin, az, za, order: int = 1):

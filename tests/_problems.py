@@ -12,7 +12,7 @@ def small_problem(nant=6, nsrc=300, ntimes=3, polarized=False, seed=1,
     (``beam_list``).
     """
     from matvis_amd import astro, synth
-    from matvis_amd.beams import GaussianBeam, grid_planes
+    from matvis_amd.beams import GaussianBeam
 
     antpos = synth.disk_array(nant, 100.0, seed)
     ra, dec = synth.random_sky(nsrc, seed + 1)
@@ -33,7 +33,11 @@ def small_problem(nant=6, nsrc=300, ntimes=3, polarized=False, seed=1,
                                                 polarized=True,
                                                 dtype=np.complex128)
                      for b in range(nbeam)]
-        preps = [grid_planes(b, freq, polarized) for b in beam_list]
+        from oracle import prepare_grid
+
+        preps = [prepare_grid(b.data_array, b.axis1_array, b.axis2_array,
+                              b.freq_array, freq, polarized, b.beam_type)
+                 for b in beam_list]
         oracle_beam = {
             "kind": "grid",
             "planes": np.stack([p[0] for p in preps]),

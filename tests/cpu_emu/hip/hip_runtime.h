@@ -36,6 +36,23 @@ constexpr hipError_t hipSuccess = 0;
 typedef struct emu_stream *hipStream_t;
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline const char *hipGetErrorString(hipError_t) { return "emulated"; }
+inline hipError_t hipGetDevice(int *d) {
+  *d = 0;
+  return hipSuccess;
+}
+// A small emulated device: 3 CUs x 2 blocks = 6 slots, rounded by callers to
+// the 8-XCD granularity, so persistent kernels loop over several items.
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 0 };
+inline hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t, int) {
+  *v = 3;
+  return hipSuccess;
+}
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n,
+                                                               const void *,
+                                                               int, size_t) {
+  *n = 2;
+  return hipSuccess;
+}
 
 // SIMT runtime (emu_runtime.cpp)
 void emu_syncthreads();
@@ -62,6 +79,17 @@ template <class T> inline T __shfl_down(T v, unsigned off, int width = 64) {
   T r = v;
   if ((unsigned)(l % width) + off < (unsigned)width)
     memcpy(&r, emu_wave_slot(l + (int)off), sizeof(T));
+  emu_wave_barrier();
+  return r;
+}
+template <class T> inline T __shfl_xor(T v, int mask, int width = 64) {
+  static_assert(sizeof(T) <= 256, "slot size");
+  memcpy(emu_wave_slot(emu_lane()), &v, sizeof(T));
+  emu_wave_barrier();
+  const int l = emu_lane(), src = l ^ mask;
+  T r = v;
+  if (src < 64 && (src / width) == (l / width) && emu_lane_live(src))
+    memcpy(&r, emu_wave_slot(src), sizeof(T));
   emu_wave_barrier();
   return r;
 }

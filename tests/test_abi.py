@@ -48,7 +48,9 @@ def test_tiles_and_workspace():
     assert lib.mv_herk_tile(7) == 0
     # m = 700 -> 11 tiles of 64 -> 66 upper tiles; 3 slices of c128 tiles
     assert lib.mv_herk_workspace_bytes(_lib.FORM_C128, 700, 3) == 3 * 66 * 64 * 64 * 16
-    assert lib.mv_herk_workspace_bytes(_lib.FORM_SPLIT_BF16, 700, 2) == 2 * 21 * 128 * 128 * 8
+    # split forms: FP32 partial tiles plus FP64 row norms of the diagonal
+    assert lib.mv_herk_workspace_bytes(_lib.FORM_SPLIT_BF16, 700, 2) == (
+        2 * 21 * 128 * 128 * 8 + 2 * 21 * 128 * 8)
     assert lib.mv_rotate_cull_workspace_bytes(0) == 256
     assert lib.mv_rotate_cull_workspace_bytes(2048 * 70) == 512
 

@@ -322,7 +322,7 @@ def test_herk_segments_and_device_length():
 
 def _split_terms(z, form, nterms, scale=1.0):
     m, k = z.shape
-    ldt = -(-k // 4) * 4
+    ldt = max(16, -(-k // 16) * 16)  # zero-padded to whole 16-column tiles
     ts = [D.zeros((m, ldt), np.int32) for _ in range(nterms)]
     t = ts + [None] * (3 - nterms)
     zd = _dev(z)
@@ -332,11 +332,11 @@ def _split_terms(z, form, nterms, scale=1.0):
     return ts, ldt
 
 
-# Bounds: the split's representation error (bf16 3 products ~7e-6, 4
-# products ~3e-6, 3 terms ~1e-8; fp16 ~1e-7 on these inputs) plus FP32
-# accumulation within one K-slice.
+# Bounds: the split's representation error with the exact diagonal (bf16
+# two terms up to 3.4e-6 at k = 77, 3 or 4 products alike; 3 terms ~5e-9;
+# fp16 ~6e-8 on these inputs) plus FP32 accumulation within one K-slice.
 @pytest.mark.parametrize("form,nterms,nprod,bound", [
-    (FORM_SPLIT_BF16, 2, 3, 3e-5), (FORM_SPLIT_BF16, 2, 4, 6e-6),
+    (FORM_SPLIT_BF16, 2, 3, 6e-6), (FORM_SPLIT_BF16, 2, 4, 6e-6),
     (FORM_SPLIT_BF16, 3, 6, 1e-6), (FORM_SPLIT_F16, 2, 3, 1e-6),
     (FORM_SPLIT_F16, 2, 4, 1e-6)])
 @pytest.mark.parametrize("m,k", [(60, 200), (300, 4000), (129, 77)])
@@ -351,8 +351,10 @@ def test_herk_split_accuracy(form, nterms, nprod, bound, m, k):
     s = f16_scale(float(np.abs(z).max())) if form == FORM_SPLIT_F16 else 1.0
     ts, ldt = _split_terms(z, form, nterms, s)
     # FP32 accumulation spans one K-slice; the slice sums are FP64
+    from matvis_amd.gpu.herk import Herk
+    cap = Herk(form, m, nterms, nprod).nslices(k)
     h, v = _run_herk(form, ts, m, ldt, 1, ldt, k, nterms, nprod,
-                     nslices=max(1, k // 256), scale=1.0 / (s * s))
+                     nslices=cap, scale=1.0 / (s * s))
     got = _vis_full(h, v, m)
     z64 = z.astype(np.complex128)
     want = z64.conj() @ z64.T
@@ -429,3 +431,27 @@ def test_vacc_sum_and_pair_expand():
     z = np.concatenate([za, zb], 1)
     want = O.vis_from_z(z, nant, nf, pairs)
     np.testing.assert_allclose(D.to_host(out), want, rtol=1e-6)
+
+
+@pytest.mark.parametrize("form,nterms,nprod", [(FORM_C128, 1, 1),
+                                               (FORM_SPLIT_BF16, 2, 3)])
+def test_herk_phases_match_single_call(form, nterms, nprod):
+    """mv_herk_compute then mv_herk_reduce equals mv_herk bit for bit."""
+    from matvis_amd.gpu.herk import Herk
+    print(f"herk phases form={form}", flush=True)
+    m, k = 150, 700
+    z = _herk_case(m, k, np.complex64, seed=5)
+    if form == FORM_C128:
+        zs, ld = [_dev(z.astype(np.complex128))], k
+    else:
+        zs, ld = _split_terms(z, form, nterms)
+    h = Herk(form, m, nterms, nprod)
+    v1, v2 = h.new_vacc(), h.new_vacc()
+    cap = h.nslices(k)
+    h.run(zs, ldz=ld, nseg=1, seg_stride=ld, seg_len=k, vacc=v1,
+          accumulate=False, nslices=cap)
+    marks = []
+    h.run(zs, ldz=ld, nseg=1, seg_stride=ld, seg_len=k, vacc=v2,
+          accumulate=False, nslices=cap, between=lambda s: marks.append(s))
+    assert len(marks) == 1
+    np.testing.assert_array_equal(_vis_full(h, v1, m), _vis_full(h, v2, m))

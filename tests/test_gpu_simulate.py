@@ -131,12 +131,13 @@ def test_single_precision_matches_oracle():
     assert e[0] < 1e-5
 
 
-# Bounds: measured on this 20-antenna, 4000-source bicubic problem the forms
-# reach about 3.5e-6 (bf16, 3 products: the dropped lo*lo term), 1.0e-6
-# (bf16, 4 products: the 2^-18 residual of a two-term bf16 split),
-# 4e-7 (bf16, 3 terms) and 1.6e-7 (fp16, 3 products).
+# Bounds on this 20-antenna, 4000-source bicubic problem: the two-term bf16
+# split leaves a 2^-18 representation residual per element, about 1e-6 here
+# with 3 or 4 products (the diagonal is summed exactly, so lo*lo only
+# matters off the diagonal, where it averages out); 3 bf16 terms and the
+# fp16 split are limited by FP32 accumulation, a few 1e-7.
 @pytest.mark.parametrize("method,terms,prods,bound", [
-    ("GPUMatMulSplitBF16", 2, 4, 2e-6), ("GPUMatMulSplitBF16", 2, 3, 1e-5),
+    ("GPUMatMulSplitBF16", 2, 4, 2e-6), ("GPUMatMulSplitBF16", 2, 3, 2e-6),
     ("GPUMatMulSplitBF16", 3, 6, 1e-6), ("GPUMatMulSplitF16", 2, 3, 1e-6)])
 def test_split_forms_match_oracle(method, terms, prods, bound):
     p = small_problem(nant=20, nsrc=4000, ntimes=2, polarized=True, beam="grid",

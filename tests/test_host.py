@@ -38,13 +38,42 @@ def _grid_beam(naz=36, nza=10, full_circle=True):
 def test_grid_planes_wraps_azimuth_and_converts_power():
     b = _grid_beam()
     planes, daz, dza, azmin, is_power = grid_planes(b, 150e6, polarized=True)
-    assert planes.shape == (2, 2, 10, 37) and not is_power
-    np.testing.assert_array_equal(planes[..., -1], planes[..., 0])
-    assert math.isclose(daz, math.radians(10)) and azmin == 0.0
+    # full circle: 3 wrapped columns on each side (pyuvdata's padding)
+    assert planes.shape == (2, 2, 10, 42) and not is_power
+    d = b.data_array[:2, :2, 0]
+    np.testing.assert_array_equal(planes[..., :3], d[..., -3:])
+    np.testing.assert_array_equal(planes[..., 3:39], d)
+    np.testing.assert_array_equal(planes[..., -3:], d[..., :3])
+    assert math.isclose(daz, math.radians(10))
+    assert math.isclose(azmin, -3 * math.radians(10))
     planes, *_, is_power = grid_planes(b, 150e6, polarized=False)
     e = b.data_array[:, 0, 0]
-    np.testing.assert_allclose(planes[0, 0, :, :36], (np.abs(e) ** 2).sum(0))
-    assert is_power and planes.shape == (1, 1, 10, 37)
+    np.testing.assert_allclose(planes[0, 0, :, 3:39], (np.abs(e) ** 2).sum(0))
+    assert is_power and planes.shape == (1, 1, 10, 42)
+
+
+@pytest.mark.parametrize("polarized", [True, False])
+@pytest.mark.parametrize("naz,full", [(36, True), (20, False), (180, True)])
+def test_grid_planes_matches_oracle_preparation(polarized, naz, full):
+    """The product's grid preparation equals the oracle's restatement of
+    _wrangle_beams + prepare_for_map_coords, frequency interpolation
+    included."""
+    import oracle as O
+    az = np.radians(np.arange(naz) * (360.0 / naz if full else 5.0))
+    if naz == 180:  # the spacing of the reference's NF_HERA_Dipole_small.fits
+        az = np.radians(np.arange(naz) * 2.005586592178771)
+    za = np.radians(np.linspace(0, 90, 10))
+    rng = np.random.default_rng(naz)
+    shape = (2, 2, 3, za.size, az.size)
+    data = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    freqs = np.array([100e6, 150e6, 200e6])
+    b = GridBeam(data, az, za, freq_array=freqs)
+    for f in (100e6, 137.5e6, 200e6):
+        planes, daz, dza, azmin, _ = grid_planes(b, f, polarized)
+        want, wdaz, wdza, wazmin = O.prepare_grid(data, az, za, freqs, f,
+                                                  polarized)
+        np.testing.assert_allclose(planes, want, rtol=1e-15, atol=0)
+        assert (daz, dza, azmin) == (wdaz, wdza, wazmin)
 
 
 def test_grid_planes_partial_azimuth_not_wrapped():
