@@ -157,6 +157,125 @@ def time_form(eng, steps, warmup, timer=None):
     return elapsed, per_step, nups
 
 
+def batched_bench(ninst: int = 256, seed: int = 7, reps: int = 5):
+    """Many instances in one grouped launch: nant from {64, 128, 350},
+    unpolarized / polarized 50/50, n_up log-uniform on [5k, 80k] above the
+    horizon, synthetic complex64 Z split to bf16 (2 terms, 3 products);
+    tiles from the autotuner; the grouped launches captured in a HIP graph
+    and replayed.  Compared with one plain complex64 HERK launch per
+    instance and one split launch per instance."""
+    import torch
+
+    import matvis_amd._device as D
+    import matvis_amd._lib as L
+    from matvis_amd.gpu import autotune
+    from matvis_amd.gpu.herk import Herk
+    from matvis_amd.gpu.herk_group import GroupedBatch
+
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    form, nt, npd = L.FORM_SPLIT_BF16, 2, 3
+    probs, zs, vis, flops = [], [], 0, 0.0
+    for i in range(ninst):
+        nant = int(rng.choice([64, 128, 350]))
+        nf = 2 if rng.random() < 0.5 else 1
+        nup = int(np.exp(rng.uniform(np.log(5000), np.log(80000))))
+        m, seg = nant * nf, -(-nup // 16) * 16
+        z = torch.randn((m, nf * seg), dtype=torch.complex64, device="cuda",
+                        generator=gen)
+        z.view(m, nf, seg)[:, :, nup:] = 0
+        terms = [D.zeros((m, nf * seg), np.int32) for _ in range(nt)]
+        L.call("mv_split_z", D.ptr(z), 0, m, nf, nup, nf * seg, seg, form, nt,
+               1.0, D.ptr(terms[0]), D.ptr(terms[1]), None, nf * seg, seg,
+               D.stream_ptr())
+        ldv = -(-m // 128) * 128
+        probs.append(dict(terms=terms, m=m, ldz=nf * seg, nseg=nf,
+                          seg_stride=seg, seg_len=nup,
+                          vacc=D.zeros((ldv, ldv), np.complex128), ldv=ldv,
+                          scale=1.0, accumulate=False))
+        zs.append(z)
+        vis += nant * nant * nf * nf
+        flops += 8.0 * nf * nup * m * (m + 1) / 2
+    torch.cuda.synchronize()
+    log(f"batched: {ninst} instances built ({flops / 1e12:.1f} Tflop useful)")
+    tune = {}
+    tiles = autotune.choose_tiles(form, nt, npd, probs, report=tune)
+    batch = GroupedBatch(form, nt, npd, probs, tiles).capture()
+    batch.replay()
+    torch.cuda.synchronize()
+    e0, e1 = (torch.cuda.Event(enable_timing=True) for _ in range(2))
+    e0.record()
+    for _ in range(reps):
+        batch.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    t_group = e0.elapsed_time(e1) / reps
+
+    def sequential(run_one):
+        run_one()
+        torch.cuda.synchronize()
+        a, b = (torch.cuda.Event(enable_timing=True) for _ in range(2))
+        a.record()
+        run_one()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b)
+
+    herks = {}
+
+    def herk_for(f, m, n1, n2):
+        return herks.setdefault((f, m), Herk(f, m, n1, n2))
+
+    def plain_all():
+        for p, z in zip(probs, zs):
+            h = herk_for(L.FORM_C64, p["m"], 1, 1)
+            h.run([z], ldz=p["ldz"], nseg=p["nseg"], seg_stride=p["seg_stride"],
+                  seg_len=p["seg_len"], vacc=p["vacc"], accumulate=False,
+                  nslices=h.nslices(p["nseg"] * p["seg_len"]))
+
+    def split_all():
+        for p in probs:
+            h = herk_for(form, p["m"], nt, npd)
+            h.run(p["terms"], ldz=p["ldz"], nseg=p["nseg"],
+                  seg_stride=p["seg_stride"], seg_len=p["seg_len"],
+                  vacc=p["vacc"], accumulate=False,
+                  nslices=h.nslices(p["nseg"] * p["seg_len"]))
+
+    t_plain = sequential(plain_all)
+    t_split = sequential(split_all)
+    # errors of the grouped result against the FP64 HERK on a sample
+    batch.replay()
+    torch.cuda.synchronize()
+    errs = []
+    for i in rng.choice(ninst, 8, replace=False):
+        p, z = probs[i], zs[i]
+        h = Herk(L.FORM_C128, p["m"])
+        v = h.new_vacc()
+        h.run([z.to(torch.complex128)], ldz=p["ldz"], nseg=p["nseg"],
+              seg_stride=p["seg_stride"], seg_len=p["seg_len"], vacc=v,
+              accumulate=False, nslices=h.nslices(p["nseg"] * p["seg_len"]))
+        m = p["m"]
+        ref = torch.triu(v[:m, :m])
+        got = torch.triu(p["vacc"][:m, :m])
+        d = torch.sqrt(torch.diagonal(ref).real.abs())
+        errs.append(float((torch.abs(got - ref) / (d[:, None] * d[None])).max()))
+    del zs, probs, batch, herks
+    torch.cuda.empty_cache()
+    return {
+        "instances": ninst, "visibilities": vis, "useful_tflop": flops / 1e12,
+        "grouped_graph_ms": t_group, "vis_per_s": vis / t_group * 1e3,
+        "useful_tflops": flops / t_group / 1e9,
+        "sequential_plain_c64_ms": t_plain,
+        "sequential_split_bf16_ms": t_split,
+        "speedup_vs_sequential_plain": t_plain / t_group,
+        "speedup_vs_sequential_split": t_split / t_group,
+        "tiles": {str(k): tiles.count(k) for k in sorted(set(tiles))},
+        "autotune_ms": tune,
+        "max_err_vs_fp64_8_instances": max(errs),
+    }
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -236,6 +355,8 @@ def main():
                     help="skip the CPU baseline (for profiling runs)")
     ap.add_argument("--forms", default="bf16,bf16x4,f16,fp64",
                     help="comma list of " + ",".join(FORMS))
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the grouped-HERK batch (256 mixed instances)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy first (the "
                          "known-size kernel that calibrates HBM byte counters)")
@@ -378,6 +499,9 @@ def main():
         "forms": results,
         "errors_normalised_max_median_elemmax_elemmedian": err,
     }
+    if not args.no_batched:
+        log("batched: grouped HERK over 256 mixed instances")
+        line["batched"] = batched_bench()
     if "fp64" in results:
         line["speedup_over_fp64"] = {
             n: results[n]["vis_per_s"] / results["fp64"]["vis_per_s"]

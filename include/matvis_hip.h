@@ -251,6 +251,44 @@ int mv_herk_reduce(int form, int nterms, int nprod, const void *z0,
                    double *vacc, int64_t ldv, int accumulate, void *workspace,
                    size_t ws_bytes, mv_stream_t stream);
 
+/* One instance of a grouped split HERK (mv_herk_grouped): plain integers
+ * and device addresses so that a binding can pack an array of them
+ * (96 bytes each; mv_herk_problem_bytes() reports the size).  Fields as in
+ * mv_herk: z = term arrays (z[2] = 0 unless 3 terms), ldz / seg_stride /
+ * seg_len in complex units (seg_len the host bound), seg_len_dev = device
+ * int32 column count or 0, vacc = device complex128 (ldv x ldv), scale,
+ * accumulate; nslices = slice capacity given to this problem's share of the
+ * workspace (mv_herk_max_slices). */
+typedef struct mv_herk_problem {
+  uint64_t z[3];
+  int64_t ldz, seg_stride, seg_len;
+  uint64_t seg_len_dev, vacc;
+  int64_t ldv;
+  double scale;
+  int32_t m, nseg, accumulate, nslices;
+} mv_herk_problem;
+
+size_t mv_herk_problem_bytes(void);
+
+/* Workspace bytes for mv_herk_grouped over these problems (host copy). */
+size_t mv_herk_grouped_workspace_bytes(int form, int tile,
+                                       const mv_herk_problem *problems,
+                                       int nprob);
+
+/* Many split-form HERKs of different sizes in one persistent launch (plus a
+ * plan and a reduce launch): every problem's (tile, K-slice) items, problem-
+ * major, are swept by one workgroup per resident slot, so small problems
+ * share the GPU instead of each launch idling most of it.  `tile` = 128 or
+ * 64 (rows per workgroup tile; 64 suits m <= 128).  problems_host is read
+ * for validation and launch geometry, problems_dev (the same array in device
+ * memory) by the kernels, so a captured graph replays with whatever the
+ * device array holds.  No reference counterpart: the reference runs one
+ * cuBLAS call per (time, frequency, chunk). */
+int mv_herk_grouped(int form, int nterms, int nprod, int tile,
+                    const mv_herk_problem *problems_host,
+                    const mv_herk_problem *problems_dev, int nprob,
+                    void *workspace, size_t ws_bytes, mv_stream_t stream);
+
 /* Sum nsrc_v accumulators elementwise into dst (all ldv x ldv c128):
  * dst = sum_i src_i.  The chunk sum of MatProd.sum_chunks
  * (src/matvis/core/matprod.py:103-115, src/matvis/gpu/matprod.py:49-51). */

@@ -59,6 +59,9 @@ SIGNATURES: dict[str, list] = {
                         _D, _P, _L, _I, _P, _S, _P],
     "mv_herk_reduce": [_I, _I, _I, _P, _P, _P, _I, _L, _I, _L, _L, _P, _I,
                        _D, _P, _L, _I, _P, _S, _P],
+    "mv_herk_problem_bytes": [],
+    "mv_herk_grouped_workspace_bytes": [_I, _I, _P, _I],
+    "mv_herk_grouped": [_I, _I, _I, _I, _P, _P, _I, _P, _S, _P],
     "mv_vacc_sum": [_P, _I, _L, _P, _L, _P],
     "mv_vis_expand": [_P, _L, _I, _I, _I, _P, _L, _P, _I, _P],
 }
@@ -66,6 +69,8 @@ SIGNATURES: dict[str, list] = {
 _RESTYPES = {
     "mv_rotate_cull_workspace_bytes": _S,
     "mv_herk_workspace_bytes": _S,
+    "mv_herk_problem_bytes": _S,
+    "mv_herk_grouped_workspace_bytes": _S,
 }
 
 _lock = threading.Lock()

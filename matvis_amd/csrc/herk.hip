@@ -280,11 +280,7 @@ __device__ __forceinline__ u32x4 swap_negate4(u32x4 v) {
                swap_negate(v[3])};
 }
 
-struct TermPtrs {
-  const uint32_t *t[3];
-};
 
-typedef uint32_t StageBuf[kST * kSRow];
 
 // The two 16-bit halves of a packed (re, im) word as floats.
 template <bool F16> __device__ __forceinline__ float lo_half(uint32_t w) {
@@ -319,16 +315,43 @@ __device__ __forceinline__ double row_norm2(const u32x4 (*r)[2]) {
   return s;
 }
 
-// 256 threads = 4 waves in a 2x2 arrangement over a 128x128 tile; each wave
-// computes a 64x64 sub-tile as 2x2 blocks of 32x32x16 MFMA, whose K = 16
-// reals are 8 from lanes 0-31 and 8 from lanes 32-63.
+// One HERK instance of a split form as the kernels read it.  `part` holds
+// `nslices` x ntiles partial tiles of TILE x TILE complex<float>, `dpart`
+// the FP64 row norms of the diagonal tiles (nslices x ntiles x TILE).
+struct SplitProblem {
+  const uint32_t *t[3];
+  int64_t ldz, seg_stride, seg_len_host;
+  const int32_t *seg_len_dev;
+  cplx<float> *part;
+  double *dpart;
+  cplx<double> *vacc;
+  int64_t ldv;
+  double scale;
+  int m, nseg, nb, ntiles, accumulate;
+};
+
+// Block shape of the split kernels for a tile side: 128 -> 256 threads as
+// 2x2 waves of 64x64; 64 -> one wave of 64x64.  Each thread stages PAIRS
+// (row, half-row) pieces of 8 words of each operand panel per K tile.
+template <int TILE> struct SplitShape {
+  static constexpr int kThreads = TILE == 128 ? 256 : 64;
+  static constexpr int kPairs = TILE * 2 / kThreads;
+};
+
+// 16-byte aligned so that every staged row piece moves as one b128 LDS access.
+template <int NT, int TILE> struct alignas(16) SplitLds {
+  uint32_t xa[NT][TILE * kSRow];
+  uint32_t xb[NT][TILE * kSRow];
+  uint32_t yb[NT][TILE * kSRow];
+};
+
+// Waves compute 64x64 sub-tiles as 2x2 blocks of 32x32x16 MFMA, whose
+// K = 16 reals are 8 from lanes 0-31 and 8 from lanes 32-63.
 //
 // Term products are grouped so that small ones share an instruction with a
 // large one: a lane half h in {0, 1} supplies its term (a_h for A, b_h for B)
 // over the same 4 complex columns, so one MFMA adds t_a0 t_b0 + t_a1 t_b1
-// inside the instruction before the single FP32 rounding into the
-// accumulator.  A lo*lo product issued on its own falls below half an ulp of
-// a long accumulation and is rounded away; grouped with hi*hi it is kept.
+// before its single FP32 rounding into the accumulator.
 //   (2 terms, 3 products): hi*hi over 8 columns, then [hi|lo] x [lo|hi]
 //   (2 terms, 4 products): [hi|lo] x [hi|lo] and [hi|lo] x [lo|hi]
 //   (3 terms, 6 products): [h|m] x [h|m], [h|m] x [m|h], [h|l] x [l|h]
@@ -337,221 +360,331 @@ __device__ __forceinline__ double row_norm2(const u32x4 (*r)[2]) {
 // One K tile per LDS stage, single-buffered: the next tile's global loads are
 // in flight in registers while the current tile is consumed.  The column
 // panel is stored twice, as X and as Y = (im, -re), so the imaginary-part
-// MFMAs read their operand instead of forming it per use.  LDS per block:
-// 3 NT x 10 KB (two blocks per CU for two terms).
-template <int NT, int MAXORD, bool F16>
-__global__ __launch_bounds__(256, 2) void herk_split_kernel(
-    TermPtrs tp, int m, int64_t ldz, int nseg, int64_t seg_stride,
-    int64_t seg_len_host, const int32_t *__restrict__ seg_len_dev, int nb,
-    int ntiles, SliceCfg cfg, cplx<float> *__restrict__ part,
-    double *__restrict__ dpart) {
-  __shared__ StageBuf Xa[NT];
-  __shared__ StageBuf Xb[NT];
-  __shared__ StageBuf Yb[NT];
-
-  const int64_t seg_len = seg_columns(seg_len_host, seg_len_dev);
-  const int kps = (int)cdiv(seg_len, (int64_t)kSK);  // K tiles per segment
-  const int ktiles = kps * nseg;
-  const int ns = active_slices(ktiles, ntiles, cfg);
-
+// MFMAs read their operand instead of forming it per use.  Threads that
+// stage a diagonal tile's rows also sum |t0 + t1 (+ t2)|^2 of them in FP64.
+template <int NT, int MAXORD, bool F16, int TILE>
+__device__ __forceinline__ void split_item(const SplitProblem &P, int slice,
+                                           int tile, int ns, int ktiles,
+                                           int kps, int64_t seg_len,
+                                           SplitLds<NT, TILE> &L) {
+  typedef SplitShape<TILE> S;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
-  // global -> LDS: thread covers row tid/2, words [(tid%2)*8, +8)
-  const int lrow = tid >> 1, lw = (tid & 1) * 8;
+  const int wr = TILE == 128 ? wave >> 1 : 0, wc = TILE == 128 ? wave & 1 : 0;
   const int lr = lane & 31, lh = lane >> 5;
-
-  for_items((int64_t)ntiles * ns, [&](int64_t it) {
-    const int slice = (int)(it / ntiles), tile = (int)(it % ntiles);
-    int bi, bj;
-    tile_coords(tile, nb, bi, bj);
-    const bool diag = bi == bj;
-    const bool active = !(diag && wr > wc) && bi * kST + wr * 64 < m &&
-                        bj * kST + wc * 64 < m;
-    const int kt0 = (int)((int64_t)slice * ktiles / ns);
-    const int kt1 = (int)((int64_t)(slice + 1) * ktiles / ns);
-    const int grow_a = bi * kST + lrow, grow_b = bj * kST + lrow;
-    const bool ok_a = grow_a < m, ok_b = grow_b < m;
-    const int64_t roff_a = (int64_t)grow_a * ldz + lw;
-    const int64_t roff_b = (int64_t)grow_b * ldz + lw;
-    double dsum = 0.0;  // diagonal tiles: sum of |t0 + t1 (+ t2)|^2 of the row
-
-    v16f acc_re[2][2], acc_im[2][2];
+  int bi, bj;
+  tile_coords(tile, P.nb, bi, bj);
+  const bool diag = bi == bj;
+  const bool active = !(diag && wr > wc) && bi * TILE + wr * 64 < P.m &&
+                      bj * TILE + wc * 64 < P.m;
+  const int kt0 = (int)((int64_t)slice * ktiles / ns);
+  const int kt1 = (int)((int64_t)(slice + 1) * ktiles / ns);
+  int lrow[S::kPairs], lw[S::kPairs];
+  bool ok_a[S::kPairs], ok_b[S::kPairs];
+  int64_t roff_a[S::kPairs], roff_b[S::kPairs];
+  double dsum[S::kPairs];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+  for (int p = 0; p < S::kPairs; ++p) {
+    const int pi = tid + p * S::kThreads;
+    lrow[p] = pi >> 1;
+    lw[p] = (pi & 1) * 8;
+    const int ga = bi * TILE + lrow[p], gb = bj * TILE + lrow[p];
+    ok_a[p] = ga < P.m;
+    ok_b[p] = gb < P.m;
+    roff_a[p] = (int64_t)ga * P.ldz + lw[p];
+    roff_b[p] = (int64_t)gb * P.ldz + lw[p];
+    dsum[p] = 0.0;
+  }
+
+  v16f acc_re[2][2], acc_im[2][2];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        acc_re[i][j] = v16f{};
-        acc_im[i][j] = v16f{};
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      acc_re[i][j] = v16f{};
+      acc_im[i][j] = v16f{};
+    }
+
+  // position of the K tile to load next: segment sg, tile kc within it
+  int sg = kt0 / (kps > 0 ? kps : 1), kc = kt0 - sg * kps;
+  u32x4 ra[S::kPairs][NT][2], rb[S::kPairs][NT][2];
+  // whole K tiles: columns [seg_len, round_up(seg_len, 16)) of every
+  // segment are zero by contract, rows >= m are not read
+  auto gload1 = [&](int64_t roff, bool row_ok, u32x4 (*r)[2]) {
+    const int64_t off = roff + sg * P.seg_stride + kc * kSK;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const uint32_t *src = P.t[t] + off;
+      if (row_ok) {
+        r[t][0] = *(const u32x4 *)src;
+        r[t][1] = *(const u32x4 *)(src + 4);
+      } else {
+        r[t][0] = u32x4{0u, 0u, 0u, 0u};
+        r[t][1] = u32x4{0u, 0u, 0u, 0u};
       }
+    }
+  };
+  auto gload = [&]() {
+#pragma unroll
+    for (int p = 0; p < S::kPairs; ++p) {
+      gload1(roff_a[p], ok_a[p], ra[p]);
+      if (!diag) gload1(roff_b[p], ok_b[p], rb[p]);
+    }
+    if (++kc == kps) {
+      kc = 0;
+      ++sg;
+    }
+  };
 
-    // position of the K tile to load next: segment sg, tile kc within it
-    int sg = kt0 / (kps > 0 ? kps : 1), kc = kt0 - sg * kps;
-    u32x4 ra[NT][2], rb[NT][2];
-    // whole K tiles: columns [seg_len, round_up(seg_len, 16)) of every
-    // segment are zero by contract, rows >= m are not read
-    auto gload1 = [&](int64_t roff, bool row_ok, u32x4 (*r)[2]) {
-      const int64_t off = roff + sg * seg_stride + kc * kSK;
+  if (kt0 < kt1) gload();
+  for (int kt = kt0; kt < kt1; ++kt) {
+    __syncthreads();  // every wave is done reading the previous tile
+#pragma unroll
+    for (int p = 0; p < S::kPairs; ++p) {
+      const int o = lrow[p] * kSRow + lw[p];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const uint32_t *src = tp.t[t] + off;
-        if (row_ok) {
-          r[t][0] = *(const u32x4 *)src;
-          r[t][1] = *(const u32x4 *)(src + 4);
+        *(u32x4 *)&L.xa[t][o] = ra[p][t][0];
+        *(u32x4 *)&L.xa[t][o + 4] = ra[p][t][1];
+        if (diag) {
+          *(u32x4 *)&L.yb[t][o] = swap_negate4(ra[p][t][0]);
+          *(u32x4 *)&L.yb[t][o + 4] = swap_negate4(ra[p][t][1]);
+          if (t == NT - 1) dsum[p] += row_norm2<NT, F16>(ra[p]);
         } else {
-          r[t][0] = u32x4{0u, 0u, 0u, 0u};
-          r[t][1] = u32x4{0u, 0u, 0u, 0u};
+          *(u32x4 *)&L.xb[t][o] = rb[p][t][0];
+          *(u32x4 *)&L.xb[t][o + 4] = rb[p][t][1];
+          *(u32x4 *)&L.yb[t][o] = swap_negate4(rb[p][t][0]);
+          *(u32x4 *)&L.yb[t][o + 4] = swap_negate4(rb[p][t][1]);
         }
       }
+    }
+    __syncthreads();
+    if (kt + 1 < kt1) gload();
+    if (!active) continue;
+    const uint32_t(*Bx)[TILE * kSRow] = diag ? L.xa : L.xb;
+    // operands of the 2x2 blocks: lane half lh reads term t_lh at word
+    // offset w_lh of its row
+    auto ld_a = [&](int t0, int t1, int w0, int w1, u32x4 (&a)[2]) {
+      const uint32_t *base = L.xa[lh ? t1 : t0];
+      const int w = lh ? w1 : w0;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        a[i] = *(const u32x4 *)&base[(wr * 64 + i * 32 + lr) * kSRow + w];
     };
-    auto gload = [&]() {
-      gload1(roff_a, ok_a, ra);
-      if (!diag) gload1(roff_b, ok_b, rb);
-      if (++kc == kps) {
-        kc = 0;
-        ++sg;
+    auto ld_b = [&](int t0, int t1, int w0, int w1, u32x4 (&b)[2],
+                    u32x4 (&y)[2]) {
+      const int t = lh ? t1 : t0, w = lh ? w1 : w0;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int o = (wc * 64 + j * 32 + lr) * kSRow + w;
+        b[j] = *(const u32x4 *)&Bx[t][o];
+        y[j] = *(const u32x4 *)&L.yb[t][o];
       }
     };
-
-    if (kt0 < kt1) gload();
-    for (int kt = kt0; kt < kt1; ++kt) {
-      __syncthreads();  // every wave is done reading the previous tile
-      {
-        const int o = lrow * kSRow + lw;
+    auto mma = [&](const u32x4 (&a)[2], const u32x4 (&b)[2],
+                   const u32x4 (&y)[2]) {
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          *(u32x4 *)&Xa[t][o] = ra[t][0];
-          *(u32x4 *)&Xa[t][o + 4] = ra[t][1];
-          if (diag) {
-            *(u32x4 *)&Yb[t][o] = swap_negate4(ra[t][0]);
-            *(u32x4 *)&Yb[t][o + 4] = swap_negate4(ra[t][1]);
-            if (t == NT - 1) dsum += row_norm2<NT, F16>(ra);
-          } else {
-            *(u32x4 *)&Xb[t][o] = rb[t][0];
-            *(u32x4 *)&Xb[t][o + 4] = rb[t][1];
-            *(u32x4 *)&Yb[t][o] = swap_negate4(rb[t][0]);
-            *(u32x4 *)&Yb[t][o + 4] = swap_negate4(rb[t][1]);
-          }
-        }
-      }
-      __syncthreads();
-      if (kt + 1 < kt1) gload();
-      if (!active) continue;
-      const StageBuf *Bx = diag ? Xa : Xb;
-      // operands of the 2x2 blocks: lane half lh reads term t_lh at word
-      // offset w_lh of its row
-      auto ld_a = [&](int t0, int t1, int w0, int w1, u32x4 (&a)[2]) {
-        const uint32_t *base = Xa[lh ? t1 : t0];
-        const int w = lh ? w1 : w0;
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-          a[i] = *(const u32x4 *)&base[(wr * 64 + i * 32 + lr) * kSRow + w];
-      };
-      auto ld_b = [&](int t0, int t1, int w0, int w1, u32x4 (&b)[2],
-                      u32x4 (&y)[2]) {
-        const int t = lh ? t1 : t0, w = lh ? w1 : w0;
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          const int o = (wc * 64 + j * 32 + lr) * kSRow + w;
-          b[j] = *(const u32x4 *)&Bx[t][o];
-          y[j] = *(const u32x4 *)&Yb[t][o];
+          acc_re[i][j] = SplitMfma<F16>::op(a[i], b[j], acc_re[i][j]);
+          acc_im[i][j] = SplitMfma<F16>::op(a[i], y[j], acc_im[i][j]);
         }
-      };
-      auto mma = [&](const u32x4 (&a)[2], const u32x4 (&b)[2],
-                     const u32x4 (&y)[2]) {
+    };
+    u32x4 a[2], b[2], y[2];
+    if constexpr (NT == 2 && MAXORD == 1) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+      for (int ks = 0; ks < 2; ++ks) {
+        ld_a(0, 0, 8 * ks, 8 * ks + 4, a);
+        ld_b(0, 0, 8 * ks, 8 * ks + 4, b, y);
+        mma(a, b, y);
+      }
 #pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            acc_re[i][j] = SplitMfma<F16>::op(a[i], b[j], acc_re[i][j]);
-            acc_im[i][j] = SplitMfma<F16>::op(a[i], y[j], acc_im[i][j]);
-          }
-      };
-      u32x4 a[2], b[2], y[2];
-      if constexpr (NT == 2 && MAXORD == 1) {
+      for (int q = 0; q < 4; ++q) {
+        ld_a(0, 1, 4 * q, 4 * q, a);
+        ld_b(1, 0, 4 * q, 4 * q, b, y);
+        mma(a, b, y);
+      }
+    } else if constexpr (NT == 2) {
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-          ld_a(0, 0, 8 * ks, 8 * ks + 4, a);
-          ld_b(0, 0, 8 * ks, 8 * ks + 4, b, y);
-          mma(a, b, y);
-        }
+      for (int q = 0; q < 4; ++q) {
+        ld_a(0, 1, 4 * q, 4 * q, a);
+        ld_b(0, 1, 4 * q, 4 * q, b, y);
+        mma(a, b, y);
+        ld_b(1, 0, 4 * q, 4 * q, b, y);
+        mma(a, b, y);
+      }
+    } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          ld_a(0, 1, 4 * q, 4 * q, a);
-          ld_b(1, 0, 4 * q, 4 * q, b, y);
-          mma(a, b, y);
-        }
-      } else if constexpr (NT == 2) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          ld_a(0, 1, 4 * q, 4 * q, a);
-          ld_b(0, 1, 4 * q, 4 * q, b, y);
-          mma(a, b, y);
-          ld_b(1, 0, 4 * q, 4 * q, b, y);
-          mma(a, b, y);
-        }
-      } else {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          ld_a(0, 1, 4 * q, 4 * q, a);
-          ld_b(0, 1, 4 * q, 4 * q, b, y);
-          mma(a, b, y);
-          ld_b(1, 0, 4 * q, 4 * q, b, y);
-          mma(a, b, y);
-          ld_a(0, 2, 4 * q, 4 * q, a);
-          ld_b(2, 0, 4 * q, 4 * q, b, y);
-          mma(a, b, y);
-        }
+      for (int q = 0; q < 4; ++q) {
+        ld_a(0, 1, 4 * q, 4 * q, a);
+        ld_b(0, 1, 4 * q, 4 * q, b, y);
+        mma(a, b, y);
+        ld_b(1, 0, 4 * q, 4 * q, b, y);
+        mma(a, b, y);
+        ld_a(0, 2, 4 * q, 4 * q, a);
+        ld_b(2, 0, 4 * q, 4 * q, b, y);
+        mma(a, b, y);
       }
     }
-    __syncthreads();  // LDS is reused by the next item
-    if (diag) {
-      // two threads share each row; lanes tid and tid ^ 1 are in one wave
-      dsum += __shfl_xor(dsum, 1, 64);
+  }
+  __syncthreads();  // LDS is reused by the next item
+  if (diag) {
+    // two threads share each row; lanes tid and tid ^ 1 are in one wave
+#pragma unroll
+    for (int p = 0; p < S::kPairs; ++p) {
+      const double s = dsum[p] + __shfl_xor(dsum[p], 1, 64);
       if ((tid & 1) == 0)
-        dpart[((int64_t)slice * ntiles + tile) * kST + lrow] = dsum;
+        P.dpart[((int64_t)slice * P.ntiles + tile) * TILE + lrow[p]] = s;
     }
+  }
 
-    // 32x32 accumulator layout: lane l, register r ->
-    //   row 8*(r/4) + 4*(l/32) + r%4, col l%32.
-    // The thread's base offset is opaque to the optimizer so that the store
-    // addresses are formed here rather than kept live across the K loop.
-    cplx<float> *out = part + ((int64_t)slice * ntiles + tile) * kST * kST;
-    int obase = (wr * 64 + 4 * lh) * kST + wc * 64 + lr;
-    asm volatile("" : "+v"(obase));
-    out += obase;
+  // 32x32 accumulator layout: lane l, register r ->
+  //   row 8*(r/4) + 4*(l/32) + r%4, col l%32.
+  // The thread's base offset is opaque to the optimizer so that the store
+  // addresses are formed here rather than kept live across the K loop.
+  cplx<float> *out = P.part + ((int64_t)slice * P.ntiles + tile) * TILE * TILE;
+  int obase = (wr * 64 + 4 * lh) * TILE + wc * 64 + lr;
+  asm volatile("" : "+v"(obase));
+  out += obase;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int row = i * 32 + 8 * (r >> 2) + (r & 3);
-          out[row * kST + j * 32] =
-              cplx<float>{acc_re[i][j][r], acc_im[i][j][r]};
-        }
+      for (int r = 0; r < 16; ++r) {
+        const int row = i * 32 + 8 * (r >> 2) + (r & 3);
+        out[row * TILE + j * 32] =
+            cplx<float>{acc_re[i][j][r], acc_im[i][j][r]};
+      }
+}
+
+// One split HERK: K tiles and the slice count from the device column count,
+// items (tile, slice) swept per XCD.
+template <int NT, int MAXORD, bool F16, int TILE>
+__global__ __launch_bounds__(SplitShape<TILE>::kThreads,
+                             TILE == 128 ? 2 : 8) void herk_split_kernel(
+    SplitProblem P, SliceCfg cfg) {
+  __shared__ SplitLds<NT, TILE> L;
+  const int64_t seg_len = seg_columns(P.seg_len_host, P.seg_len_dev);
+  const int kps = (int)cdiv(seg_len, (int64_t)kSK);  // K tiles per segment
+  const int ktiles = kps * P.nseg;
+  const int ns = active_slices(ktiles, P.ntiles, cfg);
+  for_items((int64_t)P.ntiles * ns, [&](int64_t it) {
+    split_item<NT, MAXORD, F16, TILE>(P, (int)(it / P.ntiles),
+                                      (int)(it % P.ntiles), ns, ktiles, kps,
+                                      seg_len, L);
+  });
+}
+
+// ------------------------------------------------------------ grouped
+
+// Per-launch plan of a grouped HERK, written by herk_group_plan_kernel into
+// the head of the workspace: slice counts, item and tile prefix offsets.
+struct GroupPlan {
+  int32_t *ns;        // [nprob]
+  int64_t *item_off;  // [nprob + 1]
+  int32_t *tile_off;  // [nprob + 1]
+};
+
+// Single block: every problem's internal descriptor (tile grid, partial
+// areas carved from `parts` in problem order: cap x ntiles x (ts^2 + ts)
+// x 8 bytes each), its slice count (enough slices that no FP32 chain exceeds
+// cmax K tiles, at most its capacity) and the prefix offsets.
+__global__ __launch_bounds__(256) void herk_group_plan_kernel(
+    const mv_herk_problem *__restrict__ pub, int nprob, int ts, int kc,
+    int cmax, SplitProblem *__restrict__ probs, char *__restrict__ parts,
+    GroupPlan plan) {
+  for (int p = threadIdx.x; p < nprob; p += blockDim.x) {
+    const mv_herk_problem &q = pub[p];
+    SplitProblem P;
+    for (int t = 0; t < 3; ++t) P.t[t] = (const uint32_t *)q.z[t];
+    P.ldz = q.ldz;
+    P.seg_stride = q.seg_stride;
+    P.seg_len_host = q.seg_len;
+    P.seg_len_dev = (const int32_t *)q.seg_len_dev;
+    P.vacc = (cplx<double> *)q.vacc;
+    P.ldv = q.ldv;
+    P.scale = q.scale;
+    P.m = q.m;
+    P.nseg = q.nseg;
+    P.accumulate = q.accumulate;
+    P.nb = icdiv(q.m, ts);
+    P.ntiles = P.nb * (P.nb + 1) / 2;
+    P.part = nullptr;
+    P.dpart = nullptr;
+    probs[p] = P;
+    const int64_t seg_len = seg_columns(q.seg_len, P.seg_len_dev);
+    const int ktiles = (int)(cdiv(seg_len, (int64_t)kc) * q.nseg);
+    int ns = ktiles > 0 ? icdiv(ktiles, cmax) : 1;
+    ns = ns < 1 ? 1 : (ns > q.nslices ? q.nslices : ns);
+    plan.ns[p] = ns;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t items = 0, bytes = 0;
+    int32_t tiles = 0;
+    for (int p = 0; p < nprob; ++p) {
+      SplitProblem &P = probs[p];
+      const int64_t cap = pub[p].nslices;
+      P.part = (cplx<float> *)(parts + bytes);
+      bytes += cap * P.ntiles * (int64_t)ts * ts * 8;
+      P.dpart = (double *)(parts + bytes);
+      bytes += cap * P.ntiles * (int64_t)ts * 8;
+      plan.item_off[p] = items;
+      plan.tile_off[p] = tiles;
+      items += (int64_t)P.ntiles * plan.ns[p];
+      tiles += P.ntiles;
+    }
+    plan.item_off[nprob] = items;
+    plan.tile_off[nprob] = tiles;
+  }
+}
+
+// index p with off[p] <= x < off[p + 1]
+template <typename T>
+__device__ __forceinline__ int find_segment(const T *off, int n, T x) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// All problems' items in one persistent launch, problem-major then
+// slice-major, swept per XCD as in the single-problem kernel.
+template <int NT, int MAXORD, bool F16, int TILE>
+__global__ __launch_bounds__(SplitShape<TILE>::kThreads,
+                             TILE == 128 ? 2 : 8) void herk_split_grouped_kernel(
+    const SplitProblem *__restrict__ probs, int nprob, GroupPlan plan) {
+  __shared__ SplitLds<NT, TILE> L;
+  const int64_t items = plan.item_off[nprob];
+  for_items(items, [&](int64_t it) {
+    const int p = find_segment(plan.item_off, nprob, it);
+    const SplitProblem P = probs[p];
+    const int ns = plan.ns[p];
+    const int64_t local = it - plan.item_off[p];
+    const int64_t seg_len = seg_columns(P.seg_len_host, P.seg_len_dev);
+    const int kps = (int)cdiv(seg_len, (int64_t)kSK);
+    split_item<NT, MAXORD, F16, TILE>(P, (int)(local / P.ntiles),
+                                      (int)(local % P.ntiles), ns,
+                                      kps * P.nseg, kps, seg_len, L);
   });
 }
 
 // ------------------------------------------------------------ epilogues
 
-// Sum the active slice partials of every upper tile in FP64, scale, and store
-// or add into the accumulator.
-// With `dpart` (split forms) the diagonal elements of diagonal tiles take
-// the row norms summed there instead of the MFMA sums.
+// Element e of upper tile `tile`: the sum of its `ns` slice partials in
+// FP64 (fixed order), scaled, stored or added into the accumulator.  With
+// `dpart` (split forms) the diagonal elements of diagonal tiles take the row
+// norms summed there instead of the MFMA sums.
 template <typename TP>
-__global__ __launch_bounds__(256) void herk_reduce_kernel(
+__device__ __forceinline__ void reduce_element(
     const cplx<TP> *__restrict__ part, const double *__restrict__ dpart,
-    int ntiles, int nb, int ts, int64_t seg_len_host,
-    const int32_t *__restrict__ seg_len_dev, int nseg, int kc, SliceCfg cfg,
-    double scale, cplx<double> *__restrict__ vacc, int64_t ldv,
-    int accumulate) {
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int tile = blockIdx.y;
+    int ntiles, int nb, int ts, int tile, int64_t e, int ns, double scale,
+    cplx<double> *__restrict__ vacc, int64_t ldv, int accumulate) {
   const int64_t tsz = (int64_t)ts * ts;
-  if (e >= tsz) return;
-  const int64_t seg_len = seg_columns(seg_len_host, seg_len_dev);
-  const int ktiles = (int)(cdiv(seg_len, (int64_t)kc) * nseg);
-  const int ns = active_slices(ktiles, ntiles, cfg);
   int bi, bj;
   tile_coords(tile, nb, bi, bj);
   double sr = 0.0, si = 0.0;
@@ -589,6 +722,39 @@ __global__ __launch_bounds__(256) void herk_reduce_kernel(
   } else {
     *dst = cplx<double>{sr * scale, si * scale};
   }
+}
+
+// grid (ts^2 / 256, ntiles): one thread per element of each upper tile
+template <typename TP>
+__global__ __launch_bounds__(256) void herk_reduce_kernel(
+    const cplx<TP> *__restrict__ part, const double *__restrict__ dpart,
+    int ntiles, int nb, int ts, int64_t seg_len_host,
+    const int32_t *__restrict__ seg_len_dev, int nseg, int kc, SliceCfg cfg,
+    double scale, cplx<double> *__restrict__ vacc, int64_t ldv,
+    int accumulate) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)ts * ts) return;
+  const int64_t seg_len = seg_columns(seg_len_host, seg_len_dev);
+  const int ktiles = (int)(cdiv(seg_len, (int64_t)kc) * nseg);
+  const int ns = active_slices(ktiles, ntiles, cfg);
+  reduce_element(part, dpart, ntiles, nb, ts, (int)blockIdx.y, e, ns, scale,
+                 vacc, ldv, accumulate);
+}
+
+// grid (ts^2 / 256, total tiles of all problems)
+__global__ __launch_bounds__(256) void herk_group_reduce_kernel(
+    const SplitProblem *__restrict__ probs, int nprob, int ts,
+    GroupPlan plan) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)ts * ts) return;
+  // the grid covers the tiles of the problems' largest shapes (a graph
+  // replayed with smaller ones keeps it); tiles past this plan's total idle
+  if ((int32_t)blockIdx.y >= plan.tile_off[nprob]) return;
+  const int p = find_segment(plan.tile_off, nprob, (int32_t)blockIdx.y);
+  const SplitProblem &P = probs[p];
+  reduce_element(P.part, P.dpart, P.ntiles, P.nb, ts,
+                 (int)blockIdx.y - plan.tile_off[p], e, plan.ns[p], P.scale,
+                 P.vacc, P.ldv, P.accumulate);
 }
 
 __global__ __launch_bounds__(256) void vacc_sum_kernel(
@@ -658,7 +824,7 @@ int form_cmax(int form) {
 
 // Resident workgroups of `kernel` on the current device (CUs x blocks per
 // CU), rounded down to a multiple of the XCD count; cached per device.
-template <class K> int kernel_slots(K kernel) {
+template <class K> int kernel_slots(K kernel, int block = 256) {
   static std::mutex mu;
   static std::map<std::pair<const void *, int>, int> cache;
   int dev = 0;
@@ -670,7 +836,7 @@ template <class K> int kernel_slots(K kernel) {
   int cus = 0, per = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                             dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0) !=
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0) !=
           hipSuccess)
     return -1;
   int slots = cus * (per > 0 ? per : 1);
@@ -740,13 +906,26 @@ const void *herk_kernel(int form, int nterms, int maxord) {
   }
   const bool f16 = form == MV_FORM_SPLIT_F16;
   if (nterms == 2 && maxord == 1)
-    return f16 ? (const void *)&herk_split_kernel<2, 1, true>
-               : (const void *)&herk_split_kernel<2, 1, false>;
+    return f16 ? (const void *)&herk_split_kernel<2, 1, true, kST>
+               : (const void *)&herk_split_kernel<2, 1, false, kST>;
   if (nterms == 2)
-    return f16 ? (const void *)&herk_split_kernel<2, 2, true>
-               : (const void *)&herk_split_kernel<2, 2, false>;
-  return f16 ? (const void *)&herk_split_kernel<3, 2, true>
-             : (const void *)&herk_split_kernel<3, 2, false>;
+    return f16 ? (const void *)&herk_split_kernel<2, 2, true, kST>
+               : (const void *)&herk_split_kernel<2, 2, false, kST>;
+  return f16 ? (const void *)&herk_split_kernel<3, 2, true, kST>
+             : (const void *)&herk_split_kernel<3, 2, false, kST>;
+}
+
+// The grouped kernel instantiation for (form, nterms, maxord, tile).
+template <int TILE>
+const void *grouped_kernel_t(bool f16, int nterms, int maxord) {
+  if (nterms == 2 && maxord == 1)
+    return f16 ? (const void *)&herk_split_grouped_kernel<2, 1, true, TILE>
+               : (const void *)&herk_split_grouped_kernel<2, 1, false, TILE>;
+  if (nterms == 2)
+    return f16 ? (const void *)&herk_split_grouped_kernel<2, 2, true, TILE>
+               : (const void *)&herk_split_grouped_kernel<2, 2, false, TILE>;
+  return f16 ? (const void *)&herk_split_grouped_kernel<3, 2, true, TILE>
+             : (const void *)&herk_split_grouped_kernel<3, 2, false, TILE>;
 }
 
 int plan_herk(int form, int nterms, int maxord, int m, int nslices,
@@ -868,14 +1047,28 @@ int herk_run(int phases, int form, int nterms, int nprod, const void *z0,
              "mv_herk: split terms need each segment padded to a multiple "
              "of %d columns (seg_stride %lld, ldz %lld, seg_len %lld)",
              kSK, (long long)seg_stride, (long long)ldz, (long long)seg_len);
-  TermPtrs tp = {{(const uint32_t *)z0, (const uint32_t *)z1,
-                  (const uint32_t *)z2}};
   const bool f16 = form == MV_FORM_SPLIT_F16;
-#define MV_SPLIT_LAUNCH(NT, MO, F)                                           \
-  herk_split_kernel<NT, MO, F><<<p.grid, 256, 0, st>>>(                      \
-      tp, m, ldz, nseg, seg_stride, seg_len, seg_len_dev, p.nb, p.ntiles,    \
-      p.cfg, (cplx<float> *)workspace, dpart)
   double *dpart = (double *)((char *)workspace + split_partial_bytes(m, nslices));
+  SplitProblem P;
+  P.t[0] = (const uint32_t *)z0;
+  P.t[1] = (const uint32_t *)z1;
+  P.t[2] = (const uint32_t *)z2;
+  P.ldz = ldz;
+  P.seg_stride = seg_stride;
+  P.seg_len_host = seg_len;
+  P.seg_len_dev = seg_len_dev;
+  P.part = (cplx<float> *)workspace;
+  P.dpart = dpart;
+  P.vacc = (cplx<double> *)vacc;
+  P.ldv = ldv;
+  P.scale = scale;
+  P.m = m;
+  P.nseg = nseg;
+  P.nb = p.nb;
+  P.ntiles = p.ntiles;
+  P.accumulate = accumulate;
+#define MV_SPLIT_LAUNCH(NT, MO, F)                                           \
+  herk_split_kernel<NT, MO, F, kST><<<p.grid, 256, 0, st>>>(P, p.cfg)
   if (phases & kPhaseCompute) {
     if (nterms == 2 && maxord == 1) {
       if (f16) MV_SPLIT_LAUNCH(2, 1, true);
@@ -922,6 +1115,147 @@ extern "C" int mv_herk_reduce(MV_HERK_ARGS) {
 }
 #undef MV_HERK_ARGS
 #undef MV_HERK_PASS
+
+// ------------------------------------------------------------ grouped API
+
+namespace {
+int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
+// Head of a grouped workspace: internal descriptors, then the plan arrays;
+// the partial areas follow at `parts`.
+struct GroupLayout {
+  int64_t probs, ns, item_off, tile_off, parts, total;
+};
+
+GroupLayout group_layout(int ts, const mv_herk_problem *q, int nprob) {
+  GroupLayout g;
+  g.probs = 0;
+  g.ns = align256((int64_t)nprob * sizeof(SplitProblem));
+  g.item_off = g.ns + align256((int64_t)nprob * 4);
+  g.tile_off = g.item_off + align256((int64_t)(nprob + 1) * 8);
+  g.parts = g.tile_off + align256((int64_t)(nprob + 1) * 4);
+  int64_t bytes = 0;
+  for (int i = 0; i < nprob; ++i) {
+    const int64_t nb = cdiv((int64_t)q[i].m, (int64_t)ts);
+    const int64_t ntiles = nb * (nb + 1) / 2;
+    bytes += (int64_t)q[i].nslices * ntiles * ((int64_t)ts * ts + ts) * 8;
+  }
+  g.total = g.parts + bytes;
+  return g;
+}
+
+const void *grouped_kernel(int tile, bool f16, int nterms, int maxord) {
+  return tile == 64 ? grouped_kernel_t<64>(f16, nterms, maxord)
+                    : grouped_kernel_t<kST>(f16, nterms, maxord);
+}
+}  // namespace
+
+extern "C" size_t mv_herk_problem_bytes(void) {
+  return sizeof(mv_herk_problem);
+}
+
+extern "C" size_t mv_herk_grouped_workspace_bytes(
+    int form, int tile, const mv_herk_problem *problems, int nprob) {
+  if ((form != MV_FORM_SPLIT_BF16 && form != MV_FORM_SPLIT_F16) ||
+      (tile != 64 && tile != kST) || !problems || nprob < 1)
+    return 0;
+  return (size_t)group_layout(tile, problems, nprob).total;
+}
+
+extern "C" int mv_herk_grouped(int form, int nterms, int nprod, int tile,
+                               const mv_herk_problem *problems_host,
+                               const mv_herk_problem *problems_dev, int nprob,
+                               void *workspace, size_t ws_bytes,
+                               mv_stream_t stream) {
+  MV_REQUIRE(form == MV_FORM_SPLIT_BF16 || form == MV_FORM_SPLIT_F16,
+             "mv_herk_grouped: split forms only (form %d)", form);
+  MV_REQUIRE(tile == 64 || tile == kST, "mv_herk_grouped: tile %d (64 or %d)",
+             tile, kST);
+  const int maxord = split_maxord(nterms, nprod);
+  MV_REQUIRE(maxord >= 0,
+             "mv_herk_grouped: unsupported split (nterms=%d, nprod=%d)",
+             nterms, nprod);
+  MV_REQUIRE(problems_host && problems_dev && workspace && nprob >= 1,
+             "mv_herk_grouped: null pointer or no problems");
+  int64_t total_tiles = 0, max_items = 0;
+  for (int i = 0; i < nprob; ++i) {
+    const mv_herk_problem &q = problems_host[i];
+    MV_REQUIRE(q.m >= 1 && q.nseg >= 1 && q.seg_len >= 0 && q.nslices >= 1,
+               "mv_herk_grouped: problem %d: bad sizes m=%d nseg=%d "
+               "seg_len=%lld nslices=%d", i, q.m, q.nseg,
+               (long long)q.seg_len, q.nslices);
+    MV_REQUIRE(q.z[0] && q.z[1] && (nterms < 3 || q.z[2]) && q.vacc,
+               "mv_herk_grouped: problem %d: null pointer", i);
+    const int64_t padded = cdiv(q.seg_len, (int64_t)kSK) * kSK;
+    MV_REQUIRE(q.ldz % 4 == 0 && (q.nseg == 1 || q.seg_stride % 4 == 0) &&
+                   ((q.z[0] | q.z[1] | (q.z[2] ? q.z[2] : q.z[0])) % 16) == 0,
+               "mv_herk_grouped: problem %d: terms need 16-byte aligned rows",
+               i);
+    MV_REQUIRE((q.nseg == 1 || q.seg_stride >= padded) &&
+                   q.ldz >= (int64_t)(q.nseg - 1) * q.seg_stride + padded,
+               "mv_herk_grouped: problem %d: segments must hold "
+               "round_up(seg_len, %d) columns", i, kSK);
+    const int64_t nb = cdiv((int64_t)q.m, (int64_t)tile);
+    MV_REQUIRE(q.ldv >= nb * tile,
+               "mv_herk_grouped: problem %d: ldv %lld < %lld", i,
+               (long long)q.ldv, (long long)(nb * tile));
+    total_tiles += nb * (nb + 1) / 2;
+    max_items += nb * (nb + 1) / 2 * q.nslices;
+  }
+  MV_REQUIRE(total_tiles < 65536 && max_items < (int64_t)INT32_MAX / 2,
+             "mv_herk_grouped: too many tiles (%lld) or items",
+             (long long)total_tiles);
+  const GroupLayout g = group_layout(tile, problems_host, nprob);
+  MV_REQUIRE(ws_bytes >= (size_t)g.total,
+             "mv_herk_grouped: workspace too small (%zu < %lld)", ws_bytes,
+             (long long)g.total);
+  const bool f16 = form == MV_FORM_SPLIT_F16;
+  const int threads = tile == 64 ? SplitShape<64>::kThreads
+                                 : SplitShape<kST>::kThreads;
+  const int slots = kernel_slots(grouped_kernel(tile, f16, nterms, maxord),
+                                 threads);
+  if (slots <= 0) {
+    set_error(MV_ERR_HIP, "mv_herk_grouped: occupancy query failed");
+    return MV_ERR_HIP;
+  }
+  char *ws = (char *)workspace;
+  SplitProblem *probs = (SplitProblem *)(ws + g.probs);
+  GroupPlan plan{(int32_t *)(ws + g.ns), (int64_t *)(ws + g.item_off),
+                 (int32_t *)(ws + g.tile_off)};
+  hipStream_t st = as_stream(stream);
+  herk_group_plan_kernel<<<1, 256, 0, st>>>(problems_dev, nprob, tile, kSK,
+                                            form_cmax(form), probs,
+                                            ws + g.parts, plan);
+  if (int e = check_launch("mv_herk_grouped/plan")) return e;
+  const int grid = (int)(max_items < slots ? max_items : slots);
+#define MV_GROUP_LAUNCH(NT, MO, F, T)                                        \
+  herk_split_grouped_kernel<NT, MO, F, T>                                    \
+      <<<grid, SplitShape<T>::kThreads, 0, st>>>(probs, nprob, plan)
+#define MV_GROUP_TILE(T)                                                     \
+  if (nterms == 2 && maxord == 1) {                                          \
+    if (f16) MV_GROUP_LAUNCH(2, 1, true, T);                                 \
+    else MV_GROUP_LAUNCH(2, 1, false, T);                                    \
+  } else if (nterms == 2) {                                                  \
+    if (f16) MV_GROUP_LAUNCH(2, 2, true, T);                                 \
+    else MV_GROUP_LAUNCH(2, 2, false, T);                                    \
+  } else {                                                                   \
+    if (f16) MV_GROUP_LAUNCH(3, 2, true, T);                                 \
+    else MV_GROUP_LAUNCH(3, 2, false, T);                                    \
+  }
+  if (tile == 64) {
+    MV_GROUP_TILE(64)
+  } else {
+    MV_GROUP_TILE(kST)
+  }
+#undef MV_GROUP_TILE
+#undef MV_GROUP_LAUNCH
+  if (int e = check_launch("mv_herk_grouped/split")) return e;
+  herk_group_reduce_kernel<<<dim3((unsigned)cdiv((int64_t)tile * tile,
+                                                 (int64_t)256),
+                                  (unsigned)total_tiles),
+                             256, 0, st>>>(probs, nprob, tile, plan);
+  return check_launch("mv_herk_grouped/reduce");
+}
 
 extern "C" int mv_vacc_sum(const double *src, int nsrc_v, int64_t stride,
                            double *dst, int64_t count, mv_stream_t stream) {

@@ -156,9 +156,10 @@ class VisibilityEngine:
 
     # ---------------------------------------------------------------- compute
 
-    def compute_time(self, t: int, out_dev, parity: int = 0, stream=None):
-        """Enqueue the whole of time index t; results land in out_dev
-        (nfreq, npairs, nfeed, nfeed) and the chunk counts in n_up[parity]."""
+    def stage_time(self, t: int, parity: int = 0, stream=None):
+        """Rotation, cull and compaction of time t for every chunk (and the
+        beam at the kept sources when one beam table serves all
+        frequencies); the chunk counts land in n_up[parity]."""
         if not self._ready:
             self.setup()
         co = self.coords
@@ -179,38 +180,59 @@ class VisibilityEngine:
                 self.beam_sets[0].eval(self.crd[c], self.seg, self.nsrc_alloc,
                                        self.A[c], self.seg, nup[c:c + 1],
                                        stream)
+        self._parity = parity
+
+    def stage_z(self, fi: int, c: int, stream=None):
+        """Z of frequency index fi, chunk c (after stage_time) into the Z
+        buffers; returns the HERK problem it feeds (as ``HerkGroup`` and
+        ``Herk.run`` take it)."""
+        n_dev = self.n_up[self._parity][c:c + 1]
+        if len(self.beam_sets) == 1:
+            a = self.A[c]
+        else:
+            a = self.A[0]
+            self.beam_sets[fi].eval(self.crd[c], self.seg, self.nsrc_alloc, a,
+                                    self.seg, n_dev, stream)
+        s = float(self.scales[c, fi])
         zp = [D.ptr(z) for z in self.z] + [None] * (3 - len(self.z))
-        is_f64_beam = int(self.beam_ctype == np.complex128)
-        for fi, freq in enumerate(self.freqs):
+        call("mv_zmat_fused", D.ptr(self.antpos), self.nant,
+             float(self.freqs[fi]), D.ptr(self.crd[c]), self.seg,
+             D.ptr(self.sqrt_flux[fi]), D.ptr(self.idx[c]), D.ptr(a),
+             int(self.beam_ctype == np.complex128), self.seg, self.nbeam,
+             D.ptr(self.beam_idx_dev), self.nfeed, self.nax,
+             self.nsrc_alloc, D.ptr(n_dev), self.form, self.nterms, s,
+             zp[0], zp[1], zp[2], self.ldz, self.seg, D.stream_ptr(stream))
+        return dict(terms=self.z, m=self.m, ldz=self.ldz, nseg=self.nax,
+                    seg_stride=self.seg, seg_len=self.nsrc_alloc,
+                    seg_len_dev=n_dev, vacc=self.vacc, ldv=self.herk.ldv,
+                    scale=1.0 / (s * s), accumulate=c > 0)
+
+    def run_herk(self, prob: dict, stream=None):
+        """The HERK of one staged (frequency, chunk) on its own."""
+        self.herk.run(prob["terms"], ldz=prob["ldz"], nseg=prob["nseg"],
+                      seg_stride=prob["seg_stride"], seg_len=prob["seg_len"],
+                      seg_len_dev=prob["seg_len_dev"], vacc=prob["vacc"],
+                      accumulate=prob["accumulate"], scale=prob["scale"],
+                      nslices=self.nslices, stream=stream,
+                      between=(self.herk_hooks[1] if self.herk_hooks
+                               else None))
+
+    def expand_freq(self, fi: int, out_dev, stream=None):
+        """The accumulated V of frequency fi into out_dev[fi] (API layout)."""
+        expand(self.vacc, self.herk.ldv, self.herk.tile, self.nant,
+               self.nfeed, self.pairs_dev, self.npairs, out_dev[fi], stream)
+
+    def compute_time(self, t: int, out_dev, parity: int = 0, stream=None):
+        """Enqueue the whole of time index t; results land in out_dev
+        (nfreq, npairs, nfeed, nfeed) and the chunk counts in n_up[parity]."""
+        self.stage_time(t, parity, stream)
+        for fi in range(self.nfreq):
             for c in range(self.nchunks):
-                n_dev = nup[c:c + 1]
-                if shared:
-                    a = self.A[c]
-                else:
-                    a = self.A[0]
-                    self.beam_sets[fi].eval(self.crd[c], self.seg,
-                                            self.nsrc_alloc, a, self.seg,
-                                            n_dev, stream)
-                s = float(self.scales[c, fi])
-                call("mv_zmat_fused", D.ptr(self.antpos), self.nant,
-                     float(freq), D.ptr(self.crd[c]), self.seg,
-                     D.ptr(self.sqrt_flux[fi]), D.ptr(self.idx[c]), D.ptr(a),
-                     is_f64_beam, self.seg, self.nbeam,
-                     D.ptr(self.beam_idx_dev), self.nfeed, self.nax,
-                     self.nsrc_alloc, D.ptr(n_dev), self.form, self.nterms, s,
-                     zp[0], zp[1], zp[2], self.ldz, self.seg, sp)
+                prob = self.stage_z(fi, c, stream)
                 if self.herk_hooks:
                     self.herk_hooks[0](stream)
-                self.herk.run(self.z, ldz=self.ldz, nseg=self.nax,
-                              seg_stride=self.seg, seg_len=self.nsrc_alloc,
-                              seg_len_dev=n_dev, vacc=self.vacc,
-                              accumulate=c > 0, scale=1.0 / (s * s),
-                              nslices=self.nslices, stream=stream,
-                              between=(self.herk_hooks[1] if self.herk_hooks
-                                       else None))
-            expand(self.vacc, self.herk.ldv, self.herk.tile, self.nant,
-                   self.nfeed, self.pairs_dev, self.npairs, out_dev[fi],
-                   stream)
+                self.run_herk(prob, stream)
+            self.expand_freq(fi, out_dev, stream)
         return out_dev
 
     def new_output(self):

@@ -40,7 +40,7 @@ def _resolve_matprod(matprod_method: str, precision: int, split_terms,
             split_products or cls.default_products)
 
 
-def simulate_multi(
+def build_engine(
     *,
     antpos,
     freqs,
@@ -64,11 +64,8 @@ def simulate_multi(
     split_terms: int | None = None,
     split_products: int | None = None,
 ):
-    """All frequencies at once: returns (nfreq, ntimes, npairs, nfeed, nfeed).
-
-    ``fluxes`` is (NSRCS, NFREQS).  Equivalent to calling :func:`simulate`
-    once per frequency (wrapper.py:180-199), with the rotation shared.
-    """
+    """The set-up :class:`VisibilityEngine` of a simulation and its number
+    of times; arguments as :func:`simulate_multi`."""
     if not 0 < source_buffer <= 1:
         raise ValueError("source_buffer must satisfy 0 < source_buffer <= 1")
     if not 0 < memory_buffer <= 1:
@@ -110,6 +107,17 @@ def simulate_multi(
     )
     logger.info(f"Running with {nchunks} chunks, form {form}")
     engine.setup()
+    return engine, ntimes
+
+
+def simulate_multi(**kwargs):
+    """All frequencies at once: returns (nfreq, ntimes, npairs, nfeed, nfeed).
+
+    Keywords as :func:`simulate` with ``freqs`` for ``freq`` and ``fluxes``
+    (NSRCS, NFREQS) for ``I_sky``.  Equivalent to calling :func:`simulate`
+    once per frequency (wrapper.py:180-199), with the rotation shared.
+    """
+    engine, ntimes = build_engine(**kwargs)
     return engine.run(ntimes)
 
 

@@ -1,0 +1,143 @@
+"""Grouped split HERK: many instances of different sizes in one persistent
+launch (``mv_herk_grouped``), optionally captured once in a HIP graph and
+replayed.
+
+A :class:`HerkGroup` holds one tile class (64 or 128 rows per workgroup
+tile) of a batch: the packed problem descriptors on the host and the device,
+and the workspace.  :class:`GroupedBatch` splits a batch into tile classes
+(the tile per problem comes from :mod:`matvis_amd.gpu.autotune`) and runs
+them back to back on one stream.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .. import _device as D
+from .._lib import call, lib
+from .herk import Herk, is_split
+
+#: Layout of ``mv_herk_problem`` (include/matvis_hip.h), 96 bytes.
+PROBLEM_DTYPE = np.dtype([
+    ("z", "<u8", (3,)), ("ldz", "<i8"), ("seg_stride", "<i8"),
+    ("seg_len", "<i8"), ("seg_len_dev", "<u8"), ("vacc", "<u8"),
+    ("ldv", "<i8"), ("scale", "<f8"), ("m", "<i4"), ("nseg", "<i4"),
+    ("accumulate", "<i4"), ("nslices", "<i4"),
+])
+
+
+def _addr(t) -> int:
+    return 0 if t is None else int(t.data_ptr())
+
+
+class HerkGroup:
+    """One grouped launch over problems sharing a split form and a tile.
+
+    ``problems``: list of dicts with ``terms`` (list of int32 term tensors),
+    ``m``, ``ldz``, ``nseg``, ``seg_stride``, ``seg_len`` (host bound),
+    optional ``seg_len_dev`` (device int32 tensor), ``vacc`` (complex128
+    tensor, at least ldv x ldv), ``ldv``, ``scale``, ``accumulate``.
+    The tensors must stay alive and in place while the group is used.
+    """
+
+    def __init__(self, form: int, nterms: int, nprod: int, tile: int,
+                 problems: list):
+        if not is_split(form):
+            raise ValueError("grouped HERK runs split forms only")
+        if lib().mv_herk_problem_bytes() != PROBLEM_DTYPE.itemsize:
+            raise RuntimeError("mv_herk_problem layout mismatch")
+        self.form, self.nterms, self.nprod, self.tile = form, nterms, nprod, tile
+        self.problems = problems
+        host = self._pack(problems)
+        self.host = host
+        self.dev = D.to_dev(host.view(np.uint8))
+        need = lib().mv_herk_grouped_workspace_bytes(
+            form, tile, host.ctypes.data, len(problems))
+        if need == 0:
+            raise ValueError("invalid grouped HERK problems")
+        self.ws = D.empty((need,), np.uint8)
+
+    def _pack(self, problems):
+        host = np.zeros(len(problems), PROBLEM_DTYPE)
+        for i, p in enumerate(problems):
+            terms = list(p["terms"]) + [None] * (3 - len(p["terms"]))
+            host["z"][i] = [_addr(t) for t in terms]
+            host["ldz"][i] = p["ldz"]
+            host["seg_stride"][i] = p["seg_stride"]
+            host["seg_len"][i] = p["seg_len"]
+            host["seg_len_dev"][i] = _addr(p.get("seg_len_dev"))
+            host["vacc"][i] = _addr(p["vacc"])
+            host["ldv"][i] = p["ldv"]
+            host["scale"][i] = p.get("scale", 1.0)
+            host["m"][i] = p["m"]
+            host["nseg"][i] = p["nseg"]
+            host["accumulate"][i] = int(bool(p.get("accumulate", False)))
+            host["nslices"][i] = Herk(self.form, p["m"], self.nterms,
+                                      self.nprod).nslices(
+                p["nseg"] * p["seg_len"])
+        return host
+
+    def update(self, problems):
+        """New contents for the same problem shapes (addresses, scales,
+        accumulate flags); the device copy is rewritten in place, so a
+        captured graph replays with it."""
+        host = self._pack(problems)
+        for f in ("m", "nseg", "ldz", "seg_stride", "seg_len", "ldv",
+                  "nslices"):
+            if not np.array_equal(host[f], self.host[f]):
+                raise ValueError(f"grouped problems changed shape ({f})")
+        self.problems, self.host = problems, host
+        self.dev.copy_(torch.from_numpy(host.view(np.uint8).copy()))
+
+    def run(self, stream=None):
+        call("mv_herk_grouped", self.form, self.nterms, self.nprod, self.tile,
+             self.host.ctypes.data, D.ptr(self.dev), len(self.problems),
+             D.ptr(self.ws), self.ws.numel(), D.stream_ptr(stream))
+
+
+class GroupedBatch:
+    """A batch of split HERK problems, one grouped launch per tile class;
+    ``capture()`` records the launches in a HIP graph for ``replay()``."""
+
+    def __init__(self, form: int, nterms: int, nprod: int, problems: list,
+                 tiles: list[int]):
+        self.tiles = [int(t) for t in tiles]
+        classes: dict[int, list] = {}
+        for p, t in zip(problems, self.tiles):
+            classes.setdefault(t, []).append(p)
+        self.groups = [HerkGroup(form, nterms, nprod, t, ps)
+                       for t, ps in sorted(classes.items())]
+        self.graph = None
+
+    def update(self, problems):
+        """Same problems in the same order with new contents (see
+        :meth:`HerkGroup.update`)."""
+        classes: dict[int, list] = {}
+        for p, t in zip(problems, self.tiles):
+            classes.setdefault(t, []).append(p)
+        for g in self.groups:
+            g.update(classes[g.tile])
+
+    def run(self, stream=None):
+        for g in self.groups:
+            g.run(stream)
+
+    def capture(self):
+        """Record run() in a graph (after one eager run, so the kernels'
+        occupancy queries are cached outside the capture)."""
+        self.run()
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(self.graph, stream=s):
+                self.run(torch.cuda.current_stream())
+        torch.cuda.current_stream().wait_stream(s)
+        return self
+
+    def replay(self):
+        if self.graph is None:
+            raise RuntimeError("capture() first")
+        self.graph.replay()

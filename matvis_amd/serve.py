@@ -1,0 +1,266 @@
+"""A resident visibility server on one GPU: continuous batching of
+simulation requests of uneven sizes.
+
+Requests (antenna layout, sky, frequencies, times, beam, polarization,
+precision form) are admitted while their projected device footprint fits the
+HBM budget; admitted requests stay resident (their engine keeps antenna
+positions, sky, beam tables and buffers on the device) until their last
+(time, frequency) instance is done.  Each scheduler step takes the next
+instance of every admitted request: the per-time rotation and cull, the beam
+and the fused Z kernel run per request; the HERKs of all split-BF16 /
+split-FP16 instances of the step then run as one grouped persistent launch
+(captured in a HIP graph the first time a set of requests steps together,
+replayed while that set lasts), FP64 / FP32 instances as their own plain
+launches.  Requests that arrive while others run are admitted at the next
+step, as memory frees up.
+
+Every instance's HERK depends only on its own data (the slice count comes
+from its own column count), so a request's result is the same alone and in
+any mix.
+
+Latency is submit -> result on the host; :meth:`VisibilityServer.stats`
+reports p50 / p99 and visibilities/s.
+"""
+
+from __future__ import annotations
+
+import itertools
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _device as D
+from ._lib import FORM_C64, FORM_C128
+from .gpu.herk import is_split
+
+_ids = itertools.count()
+
+
+@dataclass
+class SimRequest:
+    """One simulation; keywords as ``matvis_amd.gpu.gpu.simulate_multi``
+    (``freqs``, ``fluxes`` of shape (nsrc, nfreq), ``times``, ``skycoords``,
+    ``telescope_loc``, ``beam_list``, ``polarized``, ``precision``,
+    ``matprod_method``, ``coord_method`` ...)."""
+
+    kwargs: dict
+    rid: int = field(default_factory=lambda: next(_ids))
+    t_submit: float = 0.0
+    t_admit: float = 0.0
+    t_done: float = 0.0
+    result: np.ndarray | None = None
+    error: Exception | None = None
+    done: threading.Event = field(default_factory=threading.Event)
+
+
+def projected_bytes(engine_kwargs: dict) -> int:
+    """Device bytes a request will hold while resident: Z, accumulator,
+    HERK workspace, coordinates, beam values and the output of one time."""
+    from .gpu.herk import Herk
+
+    nant = len(engine_kwargs["antpos"])
+    nf = 2 if engine_kwargs.get("polarized") else 1
+    m = nant * nf
+    nsrc = np.asarray(engine_kwargs["fluxes"]).shape[0]
+    nfreq = np.atleast_1d(engine_kwargs["freqs"]).size
+    seg = -(-nsrc // 32) * 32
+    method = engine_kwargs.get("matprod_method", "GPUMatMul")
+    split = "Split" in method
+    zbytes = m * nf * seg * (8 if split else
+                             (16 if engine_kwargs.get("precision", 1) == 2
+                              else 8))
+    ldv = -(-m // 128) * 128
+    work = ldv * ldv * 16 * 64  # accumulator and a bound on the partials
+    out = nfreq * nant * nant * nf * nf * 16
+    return int(zbytes + work + 3 * seg * 12 + 4 * nf * nf * seg * 16 + out
+               + nsrc * nfreq * 8)
+
+
+class _Active:
+    """An admitted request and its position in (time, frequency, chunk)."""
+
+    def __init__(self, req: SimRequest, engine, ntimes: int, nbytes: int):
+        self.req, self.engine, self.ntimes, self.nbytes = req, engine, ntimes, nbytes
+        self.t = self.fi = self.c = 0
+        self.out_dev = engine.new_output()
+        e = engine
+        self.host = np.zeros((e.nfreq, ntimes, e.npairs, e.nfeed, e.nfeed),
+                             dtype=e.ctype)
+
+    @property
+    def split(self) -> bool:
+        return is_split(self.engine.form)
+
+    def advance(self) -> bool:
+        """Move past the current instance; True when the request is done."""
+        e = self.engine
+        self.c += 1
+        if self.c < e.nchunks:
+            return False
+        self.c = 0
+        self.fi += 1
+        if self.fi < e.nfreq:
+            return False
+        self.fi = 0
+        self.t += 1
+        return self.t >= self.ntimes
+
+
+class VisibilityServer:
+    """See the module docstring.  ``hbm_budget``: bytes of device memory the
+    resident requests may hold (default: 80% of free memory); ``max_active``:
+    at most this many requests resident at once; ``graphs``: capture the
+    grouped HERK launches."""
+
+    def __init__(self, hbm_budget: int | None = None, max_active: int = 8,
+                 graphs: bool = True):
+        D.require_gpu()
+        self.budget = int(hbm_budget or 0.8 * D.free_memory())
+        self.max_active = max_active
+        self.graphs = graphs
+        self.queue: list[SimRequest] = []
+        self.active: list[_Active] = []
+        self.finished: list[SimRequest] = []
+        self.used = 0
+        self.vis_done = 0
+        self._lock = threading.Lock()
+        self._batches: dict[tuple, object] = {}
+        self._thread = None
+        self._stop = threading.Event()
+
+    # ---------------------------------------------------------------- API
+
+    def submit(self, req: SimRequest) -> SimRequest:
+        req.t_submit = time.perf_counter()
+        with self._lock:
+            self.queue.append(req)
+        return req
+
+    def step(self) -> int:
+        """Admit what fits, then run one instance of every active request.
+        Returns the number of instances run."""
+        self._admit()
+        if not self.active:
+            return 0
+        split_probs, plain = [], []
+        for a in self.active:
+            e = a.engine
+            if a.fi == 0 and a.c == 0:
+                e.stage_time(a.t)
+            prob = e.stage_z(a.fi, a.c)
+            (split_probs if a.split else plain).append((a, prob))
+        for a, prob in plain:
+            a.engine.run_herk(prob)
+        if split_probs:
+            self._grouped(split_probs)
+        done = []
+        for a in self.active:
+            e = a.engine
+            if a.c == e.nchunks - 1:
+                e.expand_freq(a.fi, a.out_dev)
+                if a.fi == e.nfreq - 1:
+                    a.host[:, a.t] = D.to_host(a.out_dev)
+                    self.vis_done += e.nfreq * e.npairs * e.nfeed * e.nfeed
+            if a.advance():
+                done.append(a)
+        for a in done:
+            self._finish(a)
+        return len(split_probs) + len(plain)
+
+    def run_until_idle(self, timeout: float = 600.0):
+        t0 = time.perf_counter()
+        while (self.queue or self.active) and time.perf_counter() - t0 < timeout:
+            self.step()
+
+    def start(self):
+        """Serve from a background thread until :meth:`stop`."""
+        def loop():
+            while not self._stop.is_set():
+                if not self.step():
+                    time.sleep(0.001)
+        self._stop.clear()
+        self._thread = threading.Thread(target=loop, daemon=True)
+        self._thread.start()
+
+    def stop(self):
+        self._stop.set()
+        if self._thread is not None:
+            self._thread.join()
+
+    def stats(self) -> dict:
+        lat = np.array([r.t_done - r.t_submit for r in self.finished
+                        if r.error is None])
+        return {
+            "requests": len(self.finished),
+            "failed": sum(r.error is not None for r in self.finished),
+            "p50_s": float(np.percentile(lat, 50)) if lat.size else None,
+            "p99_s": float(np.percentile(lat, 99)) if lat.size else None,
+            "visibilities": int(self.vis_done),
+        }
+
+    # ---------------------------------------------------------------- internals
+
+    def _admit(self):
+        from .gpu.gpu import build_engine
+
+        with self._lock:
+            while self.queue and len(self.active) < self.max_active:
+                req = self.queue[0]
+                need = projected_bytes(req.kwargs)
+                if self.active and self.used + need > self.budget:
+                    break
+                self.queue.pop(0)
+                try:
+                    engine, ntimes = build_engine(**req.kwargs)
+                except Exception as exc:  # reported on the request
+                    req.error = exc
+                    req.t_done = time.perf_counter()
+                    self.finished.append(req)
+                    req.done.set()
+                    continue
+                req.t_admit = time.perf_counter()
+                self.used += need
+                self.active.append(_Active(req, engine, ntimes, need))
+
+    def _grouped(self, items):
+        from .gpu.herk_group import GroupedBatch
+
+        first = items[0][0].engine
+        form, nterms, nprod = first.form, first.nterms, first.nprod
+        # one grouped launch per split variant present in the step
+        variants: dict[tuple, list] = {}
+        for a, prob in items:
+            e = a.engine
+            variants.setdefault((e.form, e.nterms, e.nprod), []).append((a, prob))
+        for (form, nterms, nprod), its in variants.items():
+            key = (form, nterms, nprod) + tuple(a.req.rid for a, _ in its)
+            probs = [p for _, p in its]
+            batch = self._batches.get(key)
+            if batch is None:
+                tiles = [64 if p["m"] <= 128 else 128 for p in probs]
+                batch = GroupedBatch(form, nterms, nprod, probs, tiles)
+                if self.graphs:
+                    batch.capture()
+                self._batches[key] = batch
+            else:
+                batch.update(probs)
+            if self.graphs:
+                batch.replay()
+            else:
+                batch.run()
+
+    def _finish(self, a: _Active):
+        req = a.req
+        req.result = a.host
+        req.t_done = time.perf_counter()
+        with self._lock:
+            self.active.remove(a)
+            self.used -= a.nbytes
+            self.finished.append(req)
+            # graphs of sets that included this request cannot recur
+            for k in [k for k in self._batches if req.rid in k[3:]]:
+                del self._batches[k]
+        req.done.set()

@@ -1,0 +1,112 @@
+"""Grouped split HERK (mv_herk_grouped) against single-problem launches, and
+its HIP-graph capture replayed several times."""
+
+import numpy as np
+import pytest
+import torch
+
+import matvis_amd._device as D
+from matvis_amd._lib import FORM_SPLIT_BF16, FORM_SPLIT_F16, call
+
+pytestmark = pytest.mark.gpu
+
+# (m, columns per segment, segments, device column count or None)
+CASES = [(64, 1000, 1, None), (128, 3000, 2, 2500), (200, 500, 1, None),
+         (350, 4000, 2, None), (700, 2000, 2, 1900), (1, 16, 1, None),
+         (65, 17, 1, 9), (130, 70, 2, None)]
+
+
+def _problem(m, seg_len, nseg, n_dev, form, nterms, seed):
+    rng = np.random.default_rng(seed)
+    seg = -(-seg_len // 16) * 16
+    amp = np.exp(rng.uniform(-2, 0, (1, nseg * seg)) * np.log(10))
+    z = (amp * np.exp(2j * np.pi * rng.uniform(size=(m, nseg * seg))))
+    z = z.astype(np.complex64)
+    n = seg_len if n_dev is None else n_dev
+    for s in range(nseg):  # zero beyond the column count of every segment
+        z[:, s * seg + n:(s + 1) * seg] = 0
+    from matvis_amd.gpu.matprod import f16_scale
+    scale = f16_scale(float(np.abs(z).max())) if form == FORM_SPLIT_F16 else 1.0
+    zd = D.to_dev(z)
+    terms = [D.zeros((m, nseg * seg), np.int32) for _ in range(nterms)]
+    t = terms + [None] * (3 - nterms)
+    call("mv_split_z", D.ptr(zd), 0, m, nseg, seg_len, nseg * seg, seg, form,
+         nterms, scale, D.ptr(t[0]), D.ptr(t[1]), D.ptr(t[2]), nseg * seg,
+         seg, D.stream_ptr())
+    from matvis_amd.gpu.herk import Herk
+    ldv = -(-m // 128) * 128
+    return dict(terms=terms, m=m, ldz=nseg * seg, nseg=nseg, seg_stride=seg,
+                seg_len=seg_len,
+                seg_len_dev=None if n_dev is None else D.to_dev(
+                    np.array([n_dev], np.int32)),
+                vacc=D.zeros((ldv, ldv), np.complex128), ldv=ldv,
+                scale=1.0 / (scale * scale), accumulate=False, z=zd)
+
+
+def _single(p, form, nterms, nprod):
+    from matvis_amd.gpu.herk import Herk
+    h = Herk(form, p["m"], nterms, nprod)
+    v = h.new_vacc()
+    h.run(p["terms"], ldz=p["ldz"], nseg=p["nseg"], seg_stride=p["seg_stride"],
+          seg_len=p["seg_len"], seg_len_dev=p["seg_len_dev"], vacc=v,
+          accumulate=False, scale=p["scale"],
+          nslices=h.nslices(p["nseg"] * p["seg_len"]))
+    return h, v
+
+
+def _full(vacc, ldv, m, tile):
+    out = D.zeros((m * m, 1, 1), np.complex128)
+    call("mv_vis_expand", D.ptr(vacc), ldv, tile, m, 1, None, m * m,
+         D.ptr(out), 1, D.stream_ptr())
+    return D.to_host(out).reshape(m, m)
+
+
+def _nerr(got, want):
+    d = np.sqrt(np.abs(np.diag(want)))
+    d[d == 0] = 1
+    return (np.abs(got - want) / (d[:, None] * d[None])).max()
+
+
+@pytest.mark.parametrize("tile", [64, 128])
+@pytest.mark.parametrize("form,nterms,nprod", [(FORM_SPLIT_BF16, 2, 3),
+                                               (FORM_SPLIT_F16, 2, 3),
+                                               (FORM_SPLIT_BF16, 3, 6)])
+def test_grouped_matches_single_launches(tile, form, nterms, nprod):
+    from matvis_amd.gpu.herk_group import HerkGroup
+    print(f"grouped tile={tile} form={form} ({nterms},{nprod})", flush=True)
+    probs = [_problem(*c, form, nterms, seed=i) for i, c in enumerate(CASES)]
+    g = HerkGroup(form, nterms, nprod, tile, probs)
+    g.run()
+    for p, c in zip(probs, CASES):
+        h, v = _single(p, form, nterms, nprod)
+        want = _full(v, h.ldv, p["m"], h.tile)
+        got = _full(p["vacc"], p["ldv"], p["m"], tile)
+        e = _nerr(got, want)
+        print(f"  case {c}: grouped vs single {e:.2e}", flush=True)
+        # same terms, FP32 accumulation over different K-slices
+        assert e < 1e-6, (c, e)
+        # the diagonal is summed exactly in both: bit-identical up to the
+        # order of the slice sums
+        np.testing.assert_allclose(np.diag(got).real, np.diag(want).real,
+                                   rtol=1e-13, atol=0)
+
+
+def test_grouped_graph_replays():
+    from matvis_amd.gpu.herk_group import GroupedBatch
+    print("grouped graph capture", flush=True)
+    form, nterms, nprod = FORM_SPLIT_BF16, 2, 3
+    probs = [_problem(*c, form, nterms, seed=10 + i) for i, c in enumerate(CASES)]
+    tiles = [64 if p["m"] <= 128 else 128 for p in probs]
+    b = GroupedBatch(form, nterms, nprod, probs, tiles)
+    b.run()
+    torch.cuda.synchronize()
+    want = [p["vacc"].clone() for p in probs]
+    b.capture()
+    for rep in range(3):
+        for p in probs:  # eager work between replays
+            p["vacc"].zero_()
+        torch.cuda.synchronize()
+        b.replay()
+        torch.cuda.synchronize()
+        for p, w in zip(probs, want):
+            assert torch.equal(p["vacc"], w), f"replay {rep} differs"

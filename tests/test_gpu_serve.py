@@ -1,0 +1,87 @@
+"""The resident server: results identical alone and in a mix (with and
+without graphs), equal to simulate_multi, admission within the budget."""
+
+import numpy as np
+import pytest
+
+from _problems import small_problem
+from matvis_amd import astro
+from matvis_amd.gpu.gpu import simulate_multi
+from matvis_amd.serve import SimRequest, VisibilityServer
+
+pytestmark = pytest.mark.gpu
+
+
+def _kwargs(nant, nsrc, nfreq, ntimes, polarized, method, seed):
+    p = small_problem(nant=nant, nsrc=nsrc, ntimes=ntimes,
+                      polarized=polarized, seed=seed, beam="grid", order=3)
+    freqs = np.linspace(110e6, 190e6, nfreq)
+    fluxes = np.repeat(p["I_sky"][:, None], nfreq, 1) * np.linspace(
+        1.0, 0.7, nfreq)
+    return dict(antpos=p["antpos"], freqs=freqs, times=p["times"],
+                skycoords=astro.SkyCoord(p["ra"], p["dec"]),
+                telescope_loc=p["loc"], fluxes=fluxes,
+                beam_list=p["beam_list"], polarized=polarized, precision=2,
+                coord_method="CoordinateRotationMatrix",
+                coord_method_params={"rotations": p["rotations"]},
+                beam_spline_opts={"order": 3}, matprod_method=method)
+
+
+MIX = [
+    (16, 1500, 3, 2, True, "GPUMatMulSplitBF16", 1),
+    (8, 800, 2, 3, False, "GPUMatMul", 2),
+    (40, 3000, 2, 1, True, "GPUMatMulSplitBF16", 3),
+    (70, 2500, 1, 2, False, "GPUMatMulSplitF16", 4),
+    (24, 1200, 2, 2, True, "GPUMatMul", 5),
+]
+
+
+def _serve(specs, graphs=True, budget=None):
+    srv = VisibilityServer(hbm_budget=budget, graphs=graphs)
+    reqs = [srv.submit(SimRequest(_kwargs(*s))) for s in specs]
+    srv.run_until_idle(timeout=300)
+    for r in reqs:
+        assert r.error is None, r.error
+        assert r.done.is_set()
+    return srv, [r.result for r in reqs]
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_results_same_alone_and_in_mix(graphs):
+    print(f"serve mix graphs={graphs}", flush=True)
+    alone = [_serve([s], graphs)[1][0] for s in MIX]
+    srv, mixed = _serve(MIX, graphs)
+    for s, a, b in zip(MIX, alone, mixed):
+        assert np.array_equal(a, b), s
+    st = srv.stats()
+    print(f"  {st}", flush=True)
+    assert st["requests"] == len(MIX) and st["failed"] == 0
+
+
+def test_server_matches_simulate_multi():
+    print("serve vs simulate_multi", flush=True)
+    _, got = _serve(MIX)
+    for s, g in zip(MIX, got):
+        want = simulate_multi(**_kwargs(*s))
+        if s[5] == "GPUMatMul":
+            assert np.array_equal(g, want), s  # same kernels, same launches
+        else:
+            # grouped vs single-problem split launches: different K-slicing
+            import oracle as O
+            nf = 2 if s[4] else 1
+            for fi in range(g.shape[0]):
+                a, b = g[fi], want[fi]
+                if not s[4]:
+                    a, b = a[..., None, None], b[..., None, None]
+                e = O.normalized_error(a, b, s[0], nf)[0]
+                assert e < 1e-6, (s, fi, e)
+
+
+def test_admission_within_budget():
+    print("serve admission", flush=True)
+    from matvis_amd.serve import projected_bytes
+    need = [projected_bytes(_kwargs(*s)) for s in MIX]
+    budget = max(need) + min(need) // 2  # at most one large request at a time
+    srv, got = _serve(MIX, budget=budget)
+    assert srv.stats()["requests"] == len(MIX)
+    assert srv.used == 0 and not srv.active

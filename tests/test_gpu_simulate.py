@@ -242,3 +242,93 @@ def test_simulate_multi_shapes_and_dtype():
                        coord_method="CoordinateRotationERFA")
     assert v.shape == (2, 2, 25, 1, 1) and v.dtype == np.complex64
     assert np.isfinite(v).all()
+
+
+# ------------------------------------------------------------------ diffuse
+# Diffuse sky with per-antenna beams and source chunks: HEALPix pixel
+# centres, log-normal(-2, 1.5) brightness, four +-5% perturbations of one
+# E-field beam assigned round-robin (beam_idx = ant % 4), source chunks.
+
+
+def _diffuse(nside, nant, nfreq, ntimes, seed=3):
+    from matvis_amd import synth
+    ra, dec = synth.healpix_centres(nside)
+    flux = synth.lognormal_fluxes(ra.size, seed, -2.0, 1.5)
+    antpos = synth.hex_array(nant, 14.6, 0.02, seed=seed)
+    beams = [synth.harmonic_efield_beam(seed=5, naz=72, nza=19,
+                                        perturb=0.05, perturb_seed=100 + b,
+                                        dtype=np.complex128)
+             for b in range(4)]
+    freqs = np.linspace(120e6, 180e6, nfreq)
+    times = synth.times_spaced(ntimes, 10.0)
+    loc = synth.hera_location()
+    rot = astro.rotation_matrices(times, loc)
+    return dict(ra=ra, dec=dec, flux=flux, antpos=antpos, beams=beams,
+                freqs=freqs, times=times, loc=loc, rot=rot,
+                beam_idx=np.arange(nant) % 4)
+
+
+def _diffuse_gpu(d, freq, **kw):
+    opts = dict(antpos=d["antpos"], freq=freq, times=d["times"],
+                skycoords=astro.SkyCoord(d["ra"], d["dec"]),
+                telescope_loc=d["loc"], I_sky=d["flux"],
+                beam_list=d["beams"], beam_idx=d["beam_idx"], polarized=True,
+                coord_method="CoordinateRotationMatrix",
+                coord_method_params={"rotations": d["rot"]},
+                beam_spline_opts={"order": 3}, precision=2)
+    opts.update(kw)
+    return simulate(**opts)
+
+
+@pytest.mark.parametrize("method,bound", [("GPUMatMul", 1e-12),
+                                          ("GPUMatMulSplitBF16", 2e-6)])
+def test_diffuse_multibeam_chunked_matches_oracle(method, bound):
+    """nside 16 (3072 pixels), 16 antennas, 2 frequencies, 2 times, chunks of
+    512 sources, against the oracle with the same chunking."""
+    d = _diffuse(16, 16, 2, 2)
+    eci = O.point_source_crd_eq(d["ra"], d["dec"])
+    preps = [O.prepare_grid(b.data_array, b.axis1_array, b.axis2_array,
+                            b.freq_array, 150e6, True) for b in d["beams"]]
+    beam = {"kind": "grid", "planes": np.stack([p[0] for p in preps]),
+            "daz": [p[1] for p in preps], "dza": [p[2] for p in preps],
+            "azmin": [p[3] for p in preps], "order": 3}
+    for f in d["freqs"]:
+        print(f"diffuse {method} freq={f / 1e6:.0f} MHz", flush=True)
+        v = _diffuse_gpu(d, f, min_chunks=6, matprod_method=method)
+        ref = O.simulate(antpos=d["antpos"], freq=f, eci=eci,
+                         rotations=d["rot"], I_sky=d["flux"], beam=beam,
+                         polarized=True, beam_idx=d["beam_idx"], nchunks=6)
+        e = nerr(v, ref, 16, True)
+        print(f"  normalised max {e[0]:.2e}", flush=True)
+        assert e[0] < bound
+
+
+def test_diffuse_full_size_chunk_sums():
+    """HEALPix nside 128 (196,608 sources), 350 antennas, one frequency and
+    time, chunks of 16,384 sources: the FP64 result does not depend on the
+    chunking; split-BF16 stays within 1e-6; summing the per-chunk results in
+    FP32 instead of FP64 (the FP32-only variant) is stated beside them."""
+    d = _diffuse(128, 350, 1, 1)
+    f = float(d["freqs"][0])
+    print("diffuse full: fp64 12 chunks", flush=True)
+    v12 = _diffuse_gpu(d, f, min_chunks=12)
+    v1 = _diffuse_gpu(d, f, min_chunks=1)
+    e_chunk = nerr(v12, v1, 350, True)[0]
+    print("diffuse full: split-BF16 12 chunks", flush=True)
+    vb = _diffuse_gpu(d, f, min_chunks=12, matprod_method="GPUMatMulSplitBF16")
+    e_bf16 = nerr(vb, v12, 350, True)[0]
+    # FP32-only cross-chunk sum: each chunk's visibilities rounded to FP32
+    # and accumulated in FP32
+    acc32 = np.zeros(v12.shape, np.complex64)
+    n = d["ra"].size
+    for c in range(12):
+        sl = slice(c * n // 12, (c + 1) * n // 12)
+        dc = dict(d, ra=d["ra"][sl], dec=d["dec"][sl], flux=d["flux"][sl])
+        acc32 += _diffuse_gpu(dc, f, min_chunks=1).astype(np.complex64)
+    e_fp32 = nerr(acc32.astype(np.complex128), v12, 350, True)[0]
+    print(f"  chunking invariance (fp64) {e_chunk:.2e}; split-BF16 vs fp64 "
+          f"{e_bf16:.2e}; FP32-only chunk sums vs FP64 {e_fp32:.2e}",
+          flush=True)
+    assert e_chunk < 1e-12
+    assert e_bf16 < 1e-6
+    assert e_fp32 > e_chunk
