@@ -276,6 +276,170 @@ def batched_bench(ninst: int = 256, seed: int = 7, reps: int = 5):
     }
 
 
+SERVE_NANTS = (32, 128, 350, 512)
+
+
+def serving_requests(seconds: float, rate: float, seed: int = 11):
+    """A Poisson request stream: arrival offsets and request keywords.
+
+    Each request: nant from {32, 128, 350, 512} on a 14.6 m hex grid; nsrc
+    log-uniform on [1e3, 2e5] (RA uniform, sin(dec) uniform on [-90, +30]
+    deg, power-law S^-2 fluxes on [0.1, 10] Jy); nfreq uniform on [1, 64]
+    over 100-200 MHz; ntime uniform on [1, 16], 2 min apart; polarized with
+    p = 0.5; plain FP64 or split-BF16 with p = 0.5; one shared harmonic
+    E-field beam (bicubic)."""
+    from matvis_amd import astro, synth
+
+    rng = np.random.default_rng(seed)
+    n = max(1, int(rate * seconds * 2) + 16)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    arrivals = arrivals[arrivals < seconds]
+    loc = synth.hera_location()
+    beam = synth.harmonic_efield_beam(seed=4, naz=360, nza=91)
+    times = synth.times_spaced(16, 2.0)
+    rots = astro.rotation_matrices(times, loc)
+    ants = {na: synth.hex_array(na, 14.6, 0.02, seed=na) for na in SERVE_NANTS}
+    reqs = []
+    for i in range(arrivals.size):
+        nant = int(rng.choice(SERVE_NANTS))
+        nsrc = int(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
+        nfreq = int(rng.integers(1, 65))
+        ntime = int(rng.integers(1, 17))
+        pol = bool(rng.random() < 0.5)
+        fp64 = bool(rng.random() < 0.5)
+        ra, dec = synth.random_sky(nsrc, seed * 1000 + i, -90.0, 30.0)
+        flux = synth.powerlaw_fluxes(nsrc, seed * 1000 + i + 1)
+        freqs = np.linspace(100e6, 200e6, nfreq)
+        kw = dict(antpos=ants[nant], freqs=freqs, times=times[:ntime],
+                  skycoords=astro.SkyCoord(ra, dec), telescope_loc=loc,
+                  fluxes=np.repeat(flux[:, None], nfreq, 1),
+                  beam_list=[beam], polarized=pol, precision=2,
+                  coord_method="CoordinateRotationMatrix",
+                  coord_method_params={"rotations": rots[:ntime]},
+                  beam_spline_opts={"order": 3},
+                  matprod_method="GPUMatMul" if fp64 else "GPUMatMulSplitBF16")
+        label = ("fp64" if fp64 else "bf16") + ("/pol" if pol else "/unpol")
+        work = ntime * nfreq * (nant * (2 if pol else 1)) ** 2 * nsrc
+        reqs.append((float(arrivals[i]), kw, label, work))
+    return reqs
+
+
+def serving_bench(seconds: float = 20.0, rate: float = 20.0,
+                  drain: float = 20.0, seed: int = 11, max_active: int = 16):
+    """The resident server under a Poisson stream (arrivals for `seconds`,
+    then up to `drain` s to finish).  Reports latency of the finished
+    requests, lower bounds that count unfinished ones at their age, the
+    visibilities delivered per second and, for the two smallest finished
+    requests of each class, the error against the same request run alone on
+    the plain FP64 GPU path."""
+    import threading
+
+    import torch
+
+    from matvis_amd.gpu.gpu import simulate_multi
+    from matvis_amd.metrics import normalized_error
+    from matvis_amd.serve import SimRequest, VisibilityServer
+
+    stream = serving_requests(seconds, rate, seed)
+    log(f"serving: {len(stream)} requests over {seconds:.0f} s "
+        f"(Poisson {rate:g}/s)")
+    # per class, the two smallest of its first ten arrivals keep their whole
+    # result (early, so that they finish under saturation)
+    keep = set()
+    for lb in {s[2] for s in stream}:
+        idx = [i for i, s in enumerate(stream) if s[2] == lb][:10]
+        keep.update(sorted(idx, key=lambda i: stream[i][3])[:2])
+    srv = VisibilityServer(max_active=max_active)
+    reqs = [SimRequest(kw, keep=i in keep, label=lb)
+            for i, (_, kw, lb, _) in enumerate(stream)]
+    srv.start()
+    t0 = time.perf_counter()
+    sent = 0
+
+    def feeder():
+        nonlocal sent
+        for (at, _, _, _), r in zip(stream, reqs):
+            wait = t0 + at - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            srv.submit(r)
+            sent += 1
+
+    th = threading.Thread(target=feeder, daemon=True)
+    th.start()
+    last = t0
+    while True:
+        now = time.perf_counter()
+        if th.is_alive() is False and all(r.done.is_set() for r in reqs):
+            break
+        if now - t0 > seconds + drain:
+            break
+        if now - last > 5:
+            st = srv.stats()
+            log(f"serving: t={now - t0:.0f}s sent {sent} finished "
+                f"{st['requests']} active {len(srv.active)} queued "
+                f"{len(srv.queue)}")
+            last = now
+        time.sleep(0.05)
+    t_end = time.perf_counter()
+    th.join()
+    srv.stop()
+    torch.cuda.synchronize()
+    elapsed = t_end - t0
+    st = srv.stats()
+    # unfinished requests count at their age when the run ended
+    ages = np.array([(r.t_done if r.done.is_set() else t_end) - r.t_submit
+                     for r in reqs])
+    out = {
+        "arrival_window_s": seconds, "rate_per_s": rate, "drain_s": drain,
+        "elapsed_s": elapsed, "submitted": len(reqs),
+        "finished": st["requests"], "failed": st["failed"],
+        "unfinished": sum(not r.done.is_set() for r in reqs),
+        "p50_s": st["p50_s"], "p99_s": st["p99_s"],
+        "p50_s_all_lower_bound": float(np.percentile(ages, 50)),
+        "p99_s_all_lower_bound": float(np.percentile(ages, 99)),
+        "vis_per_s": st["visibilities_all"] / elapsed,
+        "finished_per_s": st["requests"] / elapsed,
+        "by_class": st.get("by_label", {}),
+        "max_active": max_active,
+    }
+    del srv
+    torch.cuda.empty_cache()
+
+    def nerr(got, ref, nant, nf):
+        a = got.reshape(-1, nant * nant, nf, nf)
+        b = ref.reshape(-1, nant * nant, nf, nf)
+        e = [normalized_error(a[j], b[j], nant, nf) for j in range(a.shape[0])]
+        return max(x[0] for x in e), [x[1] for x in e]
+
+    # served result vs the request alone on the plain FP64 path; for split
+    # requests also the request alone through simulate_multi in its own
+    # form, which separates the form's rounding from anything the mix adds
+    errs = {}
+    for i in sorted(keep):
+        r = reqs[i]
+        if r.result is None:
+            continue
+        nant = len(r.kwargs["antpos"])
+        nf = 2 if r.kwargs["polarized"] else 1
+        ref = simulate_multi(**dict(r.kwargs, matprod_method="GPUMatMul"))
+        emax, emed = nerr(r.result, ref, nant, nf)
+        cls = errs.setdefault(r.label, {"max": 0.0, "median": [],
+                                        "requests": []})
+        cls["max"] = max(cls["max"], emax)
+        cls["median"] += emed
+        desc = {"nant": nant, "nsrc": int(r.kwargs["fluxes"].shape[0]),
+                "instances": int(r.result.shape[0] * r.result.shape[1])}
+        if r.kwargs["matprod_method"] != "GPUMatMul":
+            alone = simulate_multi(**r.kwargs)
+            desc["alone_same_form_max"] = nerr(alone, ref, nant, nf)[0]
+        cls["requests"].append(desc)
+    for cls in errs.values():
+        cls["median"] = float(np.median(cls["median"]))
+    out["error_vs_fp64_alone_normalised"] = errs
+    return out
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -357,6 +521,14 @@ def main():
                     help="comma list of " + ",".join(FORMS))
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the grouped-HERK batch (256 mixed instances)")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the serving run (Poisson request stream)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="arrival window of the serving run")
+    ap.add_argument("--serve-rate", type=float, default=20.0,
+                    help="mean requests per second of the serving run")
+    ap.add_argument("--serve-drain", type=float, default=20.0,
+                    help="seconds after the window for requests to finish")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy first (the "
                          "known-size kernel that calibrates HBM byte counters)")
@@ -371,6 +543,7 @@ def main():
 
     import matvis_amd._lib as L
     from matvis_amd.coordinates import point_source_crd_eq
+    from matvis_amd.metrics import normalized_error
 
     torch.cuda.set_device(0)
     if args.copy_probe:
@@ -427,8 +600,7 @@ def main():
         torch.cuda.empty_cache()
 
     def nerr(a, b):
-        import oracle as O
-        return O.normalized_error(a, b, nant, 2)
+        return normalized_error(a, b, nant, 2)
 
     err = {}
     if "fp64" in outs:
@@ -502,6 +674,9 @@ def main():
     if not args.no_batched:
         log("batched: grouped HERK over 256 mixed instances")
         line["batched"] = batched_bench()
+    if not args.no_serving:
+        line["serving"] = serving_bench(args.serve_seconds, args.serve_rate,
+                                        args.serve_drain)
     if "fp64" in results:
         line["speedup_over_fp64"] = {
             n: results[n]["vis_per_s"] / results["fp64"]["vis_per_s"]

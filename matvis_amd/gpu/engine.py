@@ -39,7 +39,8 @@ class VisibilityEngine:
     def __init__(self, *, antpos, freqs, coords, fluxes, beam_list, beam_idx,
                  polarized: bool, precision: int, form: int, nterms: int = 1,
                  nprod: int = 1, order: int = 1, nchunks: int, chunk_size: int,
-                 nsrc_alloc: int, antpairs=None):
+                 nsrc_alloc: int, antpairs=None, beam_cache: dict | None = None,
+                 beam_src=None):
         self.antpos_h = np.asarray(antpos, dtype=np.float64)
         self.nant = self.antpos_h.shape[0]
         self.freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
@@ -72,6 +73,10 @@ class VisibilityEngine:
                                       or self.pairs_h.max() >= self.nant):
                 raise ValueError("antpairs contains an antenna index out of range")
             self.npairs = len(self.pairs_h)
+        #: device beam tables shared between engines (a resident server's
+        #: cache), keyed by the caller's beam objects ``beam_src``
+        self.beam_cache = beam_cache
+        self.beam_src = list(beam_src) if beam_src is not None else None
         self._ready = False
         #: optional (before, after) callables taking the stream, run around
         #: every HERK compute kernel (event timing of the dominant kernel;
@@ -126,19 +131,27 @@ class VisibilityEngine:
         self._ready = True
 
     def _make_beam_sets(self):
-        sets, keys = [], []
-        for f in self.freqs:
-            key = self._beam_key(f)
-            if keys and key == keys[0]:
-                continue
-            keys.append(key)
-            sets.append(DeviceBeamSet(self.beam_list, f, self.polarized,
-                                      self.beam_ctype, self.order))
-        if len(sets) > 1:
-            sets = [DeviceBeamSet(self.beam_list, f, self.polarized,
-                                  self.beam_ctype, self.order)
-                    for f in self.freqs]
-        return sets
+        keys = [self._beam_key(f) for f in self.freqs]
+        if all(k == keys[0] for k in keys):
+            return [self._beam_set(self.freqs[0])]
+        return [self._beam_set(f) for f in self.freqs]
+
+    def _beam_set(self, f):
+        """Tables at frequency f, from the shared cache when one is given
+        (keyed by the caller's beam objects, which the cache keeps alive so
+        that their ids stay unique, and by everything the tables depend on)."""
+        if self.beam_cache is None or self.beam_src is None:
+            return DeviceBeamSet(self.beam_list, f, self.polarized,
+                                 self.beam_ctype, self.order)
+        fkey = tuple(k[2:] if k[0] == "grid" else k for k in self._beam_key(f))
+        key = (tuple(id(b) for b in self.beam_src), self.polarized,
+               np.dtype(self.beam_ctype).str, self.order, fkey)
+        hit = self.beam_cache.get(key)
+        if hit is None:
+            hit = (DeviceBeamSet(self.beam_list, f, self.polarized,
+                                 self.beam_ctype, self.order), self.beam_src)
+            self.beam_cache[key] = hit
+        return hit[0]
 
     def _beam_key(self, f):
         """Frequency-response key: equal keys give identical beam tables."""

@@ -63,9 +63,12 @@ def build_engine(
     memory_buffer: float = 0.9,
     split_terms: int | None = None,
     split_products: int | None = None,
+    beam_cache: dict | None = None,
 ):
     """The set-up :class:`VisibilityEngine` of a simulation and its number
-    of times; arguments as :func:`simulate_multi`."""
+    of times; arguments as :func:`simulate_multi`.  ``beam_cache``: a dict
+    shared between engines so that device beam tables of the same beam
+    objects are built once (see ``VisibilityEngine``)."""
     if not 0 < source_buffer <= 1:
         raise ValueError("source_buffer must satisfy 0 < source_buffer <= 1")
     if not 0 < memory_buffer <= 1:
@@ -104,6 +107,7 @@ def build_engine(
         precision=precision, form=form, nterms=nterms, nprod=nprod,
         order=spline_order(beam_spline_opts), nchunks=nchunks,
         chunk_size=npixc, nsrc_alloc=nsrc_alloc, antpairs=antpairs,
+        beam_cache=beam_cache, beam_src=beam_list,
     )
     logger.info(f"Running with {nchunks} chunks, form {form}")
     engine.setup()

@@ -18,8 +18,13 @@ Every instance's HERK depends only on its own data (the slice count comes
 from its own column count), so a request's result is the same alone and in
 any mix.
 
+Results of each time go to the host through pinned buffers, double-buffered
+per request, so the device runs ahead of the host by up to two times of every
+resident request; device beam tables are shared between requests that use the
+same beam objects.
+
 Latency is submit -> result on the host; :meth:`VisibilityServer.stats`
-reports p50 / p99 and visibilities/s.
+reports p50 / p99 and visibilities/s, overall and per request label.
 """
 
 from __future__ import annotations
@@ -28,6 +33,7 @@ import itertools
 import threading
 import time
 from dataclasses import dataclass, field
+from typing import Callable
 
 import numpy as np
 import torch
@@ -47,10 +53,19 @@ class SimRequest:
     ``matprod_method``, ``coord_method`` ...)."""
 
     kwargs: dict
+    #: keep the whole (nfreq, ntimes, npairs, nfeed, nfeed) result in
+    #: ``result``; with False only ``on_time`` sees the data
+    keep: bool = True
+    #: called as on_time(request, t, values) with the (nfreq, npairs, nfeed,
+    #: nfeed) result of time t (a buffer reused afterwards: copy to keep)
+    on_time: Callable | None = None
+    #: free-form class name for per-class statistics
+    label: str = ""
     rid: int = field(default_factory=lambda: next(_ids))
     t_submit: float = 0.0
     t_admit: float = 0.0
     t_done: float = 0.0
+    visibilities: int = 0
     result: np.ndarray | None = None
     error: Exception | None = None
     done: threading.Event = field(default_factory=threading.Event)
@@ -80,15 +95,23 @@ def projected_bytes(engine_kwargs: dict) -> int:
 
 
 class _Active:
-    """An admitted request and its position in (time, frequency, chunk)."""
+    """An admitted request, its position in (time, frequency, chunk) and its
+    double-buffered device -> host path."""
 
     def __init__(self, req: SimRequest, engine, ntimes: int, nbytes: int):
         self.req, self.engine, self.ntimes, self.nbytes = req, engine, ntimes, nbytes
         self.t = self.fi = self.c = 0
-        self.out_dev = engine.new_output()
         e = engine
-        self.host = np.zeros((e.nfreq, ntimes, e.npairs, e.nfeed, e.nfeed),
-                             dtype=e.ctype)
+        self.out_dev = [e.new_output(), e.new_output()]
+        shape = tuple(self.out_dev[0].shape)
+        self.pin = [torch.empty(shape, dtype=self.out_dev[0].dtype,
+                                pin_memory=True) for _ in range(2)]
+        self.nup_pin = torch.empty((2, e.nchunks), dtype=torch.int32,
+                                   pin_memory=True)
+        self.ready = [torch.cuda.Event(), torch.cuda.Event()]
+        self.pending: list[tuple[int, int]] = []  # (time, buffer)
+        self.host = (np.zeros((e.nfreq, ntimes, e.npairs, e.nfeed, e.nfeed),
+                              dtype=e.ctype) if req.keep else None)
 
     @property
     def split(self) -> bool:
@@ -107,6 +130,40 @@ class _Active:
         self.fi = 0
         self.t += 1
         return self.t >= self.ntimes
+
+    def ship(self, t: int, copy_stream):
+        """Enqueue the copy of time t's result to the host on
+        `copy_stream`, after the compute stream's work so far."""
+        k = t % 2
+        if len(self.pending) == 2:
+            self.deliver()
+        e = self.engine
+        computed = torch.cuda.Event()
+        computed.record()
+        copy_stream.wait_event(computed)
+        with torch.cuda.stream(copy_stream):
+            self.pin[k].copy_(self.out_dev[k], non_blocking=True)
+            self.nup_pin[k].copy_(e.n_up[k], non_blocking=True)
+            self.ready[k].record(copy_stream)
+        self.pending.append((t, k))
+
+    def stage_time(self):
+        """Rotation and cull of the current time, once the copy out of the
+        buffers it reuses (those of time t - 2) has finished."""
+        if self.t >= 2:
+            torch.cuda.current_stream().wait_event(self.ready[self.t % 2])
+        self.engine.stage_time(self.t, parity=self.t % 2)
+
+    def deliver(self):
+        """Hand the oldest shipped time to the request (waits for its copy)."""
+        t, k = self.pending.pop(0)
+        self.ready[k].synchronize()
+        self.engine.check_counts(self.nup_pin[k].numpy())
+        vals = self.pin[k].numpy()
+        if self.host is not None:
+            self.host[:, t] = vals
+        if self.req.on_time is not None:
+            self.req.on_time(self.req, t, vals)
 
 
 class VisibilityServer:
@@ -128,6 +185,8 @@ class VisibilityServer:
         self.vis_done = 0
         self._lock = threading.Lock()
         self._batches: dict[tuple, object] = {}
+        self._beams: dict = {}
+        self._copy = torch.cuda.Stream()
         self._thread = None
         self._stop = threading.Event()
 
@@ -145,27 +204,42 @@ class VisibilityServer:
         self._admit()
         if not self.active:
             return 0
-        split_probs, plain = [], []
+        split_probs, plain, failed = [], [], []
         for a in self.active:
-            e = a.engine
-            if a.fi == 0 and a.c == 0:
-                e.stage_time(a.t)
-            prob = e.stage_z(a.fi, a.c)
-            (split_probs if a.split else plain).append((a, prob))
+            try:
+                e = a.engine
+                if a.fi == 0 and a.c == 0:
+                    a.stage_time()
+                prob = e.stage_z(a.fi, a.c)
+                (split_probs if a.split else plain).append((a, prob))
+            except Exception as exc:  # reported on the request
+                failed.append((a, exc))
         for a, prob in plain:
             a.engine.run_herk(prob)
         if split_probs:
             self._grouped(split_probs)
         done = []
         for a in self.active:
+            if any(a is f for f, _ in failed):
+                continue
             e = a.engine
-            if a.c == e.nchunks - 1:
-                e.expand_freq(a.fi, a.out_dev)
-                if a.fi == e.nfreq - 1:
-                    a.host[:, a.t] = D.to_host(a.out_dev)
-                    self.vis_done += e.nfreq * e.npairs * e.nfeed * e.nfeed
-            if a.advance():
-                done.append(a)
+            try:
+                if a.c == e.nchunks - 1:
+                    e.expand_freq(a.fi, a.out_dev[a.t % 2])
+                    if a.fi == e.nfreq - 1:
+                        a.ship(a.t, self._copy)
+                        n = e.nfreq * e.npairs * e.nfeed * e.nfeed
+                        a.req.visibilities += n
+                        self.vis_done += n
+                if a.advance():
+                    while a.pending:
+                        a.deliver()
+                    done.append(a)
+            except Exception as exc:
+                failed.append((a, exc))
+        for a, exc in failed:
+            a.req.error = exc
+            done.append(a)
         for a in done:
             self._finish(a)
         return len(split_probs) + len(plain)
@@ -191,39 +265,54 @@ class VisibilityServer:
             self._thread.join()
 
     def stats(self) -> dict:
-        lat = np.array([r.t_done - r.t_submit for r in self.finished
-                        if r.error is None])
-        return {
-            "requests": len(self.finished),
-            "failed": sum(r.error is not None for r in self.finished),
-            "p50_s": float(np.percentile(lat, 50)) if lat.size else None,
-            "p99_s": float(np.percentile(lat, 99)) if lat.size else None,
-            "visibilities": int(self.vis_done),
-        }
+        """p50 / p99 latency (s) and visibilities of the finished requests,
+        overall and per label."""
+        def summary(reqs):
+            lat = np.array([r.t_done - r.t_submit for r in reqs
+                            if r.error is None])
+            return {
+                "requests": len(reqs),
+                "failed": sum(r.error is not None for r in reqs),
+                "p50_s": float(np.percentile(lat, 50)) if lat.size else None,
+                "p99_s": float(np.percentile(lat, 99)) if lat.size else None,
+                "visibilities": int(sum(r.visibilities for r in reqs)),
+            }
+
+        out = summary(self.finished)
+        out["visibilities_all"] = int(self.vis_done)
+        labels = sorted({r.label for r in self.finished if r.label})
+        if labels:
+            out["by_label"] = {lb: summary([r for r in self.finished
+                                            if r.label == lb])
+                               for lb in labels}
+        return out
 
     # ---------------------------------------------------------------- internals
 
     def _admit(self):
         from .gpu.gpu import build_engine
 
-        with self._lock:
-            while self.queue and len(self.active) < self.max_active:
-                req = self.queue[0]
-                need = projected_bytes(req.kwargs)
+        while len(self.active) < self.max_active:
+            with self._lock:
+                if not self.queue:
+                    return
+                need = projected_bytes(self.queue[0].kwargs)
                 if self.active and self.used + need > self.budget:
-                    break
-                self.queue.pop(0)
-                try:
-                    engine, ntimes = build_engine(**req.kwargs)
-                except Exception as exc:  # reported on the request
-                    req.error = exc
-                    req.t_done = time.perf_counter()
+                    return
+                req = self.queue.pop(0)
+            try:
+                engine, ntimes = build_engine(**req.kwargs,
+                                              beam_cache=self._beams)
+            except Exception as exc:  # reported on the request
+                req.error = exc
+                req.t_done = time.perf_counter()
+                with self._lock:
                     self.finished.append(req)
-                    req.done.set()
-                    continue
-                req.t_admit = time.perf_counter()
-                self.used += need
-                self.active.append(_Active(req, engine, ntimes, need))
+                req.done.set()
+                continue
+            req.t_admit = time.perf_counter()
+            self.used += need
+            self.active.append(_Active(req, engine, ntimes, need))
 
     def _grouped(self, items):
         from .gpu.herk_group import GroupedBatch
@@ -254,7 +343,7 @@ class VisibilityServer:
 
     def _finish(self, a: _Active):
         req = a.req
-        req.result = a.host
+        req.result = a.host if req.error is None else None
         req.t_done = time.perf_counter()
         with self._lock:
             self.active.remove(a)

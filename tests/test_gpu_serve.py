@@ -85,3 +85,65 @@ def test_admission_within_budget():
     srv, got = _serve(MIX, budget=budget)
     assert srv.stats()["requests"] == len(MIX)
     assert srv.used == 0 and not srv.active
+
+
+def test_streamed_results_and_labels():
+    """keep=False with an on_time consumer sees the same values as a kept
+    result; statistics per label; device beam tables shared."""
+    print("serve streamed", flush=True)
+    _, kept = _serve(MIX)
+    srv = VisibilityServer()
+    got = {}
+
+    def consume(req, t, vals):
+        got.setdefault(req.rid, {})[t] = vals.copy()
+
+    reqs = [srv.submit(SimRequest(_kwargs(*s), keep=False, on_time=consume,
+                                  label=s[5])) for s in MIX]
+    srv.run_until_idle(timeout=300)
+    for r, want in zip(reqs, kept):
+        assert r.error is None and r.result is None
+        times = got[r.rid]
+        assert sorted(times) == list(range(want.shape[1]))
+        for t, v in times.items():
+            assert np.array_equal(v, want[:, t])
+    st = srv.stats()
+    assert set(st["by_label"]) == {s[5] for s in MIX}
+    assert sum(v["requests"] for v in st["by_label"].values()) == len(MIX)
+    # one grid beam object per request: tables keyed by it, not rebuilt for
+    # a second request on the same beam
+    n_tables = len(srv._beams)
+    again = srv.submit(SimRequest(reqs[0].kwargs))
+    srv.run_until_idle(timeout=300)
+    assert again.error is None and len(srv._beams) == n_tables
+
+
+def test_requests_admitted_while_others_run():
+    """Continuous batching from the background thread: requests submitted
+    while a long one is resident join the running set before it is done,
+    and every result equals the one computed alone."""
+    import time
+
+    print("serve background", flush=True)
+    long_spec = (40, 3000, 4, 8, True, "GPUMatMulSplitBF16", 9)
+    specs = [long_spec] + MIX
+    kws = [_kwargs(*s) for s in specs]
+    alone = [_serve([s])[1][0] for s in specs]
+    srv = VisibilityServer()
+    srv.start()
+    try:
+        first = srv.submit(SimRequest(kws[0]))
+        t0 = time.perf_counter()
+        while not any(a.req is first for a in list(srv.active)):
+            assert time.perf_counter() - t0 < 60
+            time.sleep(0.0005)
+        rest = [srv.submit(SimRequest(k)) for k in kws[1:]]
+        for r in [first] + rest:
+            assert r.done.wait(300), "request did not finish"
+    finally:
+        srv.stop()
+    for r, a, s in zip([first] + rest, alone, specs):
+        assert r.error is None, r.error
+        assert np.array_equal(r.result, a), s
+    # the later requests ran while the long one was resident
+    assert all(r.t_admit < first.t_done for r in rest)

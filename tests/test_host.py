@@ -217,3 +217,25 @@ def test_matprod_antpairs_validation():
         GPUMatMul(1, 1, 3, np.array([[0, 3]]))
     mp = GPUMatMul(2, 2, 3, None, precision=2)
     assert mp.npairs == 9 and mp.antpairs[4].tolist() == [1, 1]
+
+
+@pytest.mark.parametrize("nf", [1, 2])
+def test_product_error_metric_matches_oracle(nf):
+    """The metric the bench reports equals the oracle's definition, and a
+    planted error on one baseline shows at the scale the autos give it."""
+    import oracle as O
+    from matvis_amd.metrics import normalized_error
+
+    rng = np.random.default_rng(3)
+    nant = 7
+    z = rng.normal(size=(nant * nf, 40)) + 1j * rng.normal(size=(nant * nf, 40))
+    want = O.vis_from_z(z, nant, nf)[None]
+    got = want.copy()
+    got[0, 2 * nant + 5, 0, nf - 1] += 1e-3
+    a = normalized_error(got, want, nant, nf)
+    b = O.normalized_error(got, want, nant, nf)
+    np.testing.assert_allclose(a, b, rtol=1e-15, atol=0)
+    auto = lambda i, p: abs(want[0, i * nant + i, p, p])  # noqa: E731
+    # element (2, 5, 0, nf-1) pairs conj(Z[2, nf-1]) with Z[5, 0]
+    np.testing.assert_allclose(a[0], 1e-3 / np.sqrt(auto(2, nf - 1) * auto(5, 0)),
+                               rtol=1e-12)
