@@ -99,6 +99,16 @@ __device__ __forceinline__ void for_items(int64_t items, F &&body) {
   for (int64_t it = lo + j; it < hi; it += workers) body(it);
 }
 
+// Workgroup barrier ordering LDS accesses only.  __syncthreads() is a
+// release/acquire fence for all address spaces and so also waits for every
+// global load in flight; the K loops keep the next tiles' loads in flight
+// across their barriers, and wait for them where the registers are used.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ------------------------------------------------------------ plain forms
 
 template <typename T> struct PlainMfma;
@@ -192,7 +202,7 @@ __global__ __launch_bounds__(256) void herk_plain_kernel(
         lstore(Bs[0], rb);
       }
     }
-    __syncthreads();
+    lds_barrier();
 
     for (int64_t kt = kt0; kt < kt1; ++kt) {
       const int buf = (int)((kt - kt0) & 1);
@@ -232,7 +242,7 @@ __global__ __launch_bounds__(256) void herk_plain_kernel(
         lstore(As[buf ^ 1], ra);
         if (!diag) lstore(Bs[buf ^ 1], rb);
       }
-      __syncthreads();
+      lds_barrier();
     }
 
     // 16x16 accumulator: lane l = 16 g + rl, register r -> row
@@ -255,7 +265,6 @@ __global__ __launch_bounds__(256) void herk_plain_kernel(
 
 constexpr int kST = 128;  // tile side, complex rows
 constexpr int kSK = 16;   // complex columns per LDS stage (one K tile)
-constexpr int kSRow = kSK + 4;  // LDS row stride in 32-bit words (80 bytes)
 
 template <bool F16> struct SplitMfma;
 template <> struct SplitMfma<false> {
@@ -338,12 +347,50 @@ template <int TILE> struct SplitShape {
   static constexpr int kPairs = TILE * 2 / kThreads;
 };
 
-// 16-byte aligned so that every staged row piece moves as one b128 LDS access.
-template <int NT, int TILE> struct alignas(16) SplitLds {
-  uint32_t xa[NT][TILE * kSRow];
-  uint32_t xb[NT][TILE * kSRow];
-  uint32_t yb[NT][TILE * kSRow];
+// LDS position of word w (a multiple of 4) of row r in a K-tile panel of
+// 16 words a row: the row's 16-byte chunk c is stored at chunk c ^ swz(r),
+// so 16 (or 8) consecutive rows accessed at one chunk, as the operand reads
+// and the staging stores do, fall in 16 (or 8) distinct 16-byte bank groups.
+__device__ __forceinline__ int swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+__device__ __forceinline__ int lds_word(int r, int w) {
+  return r * kSK + (((w >> 2) ^ swz(r)) << 2);
+}
+
+// 16 bytes from global memory per lane straight into LDS: lane l's bytes
+// land at byte 16 l of the wave-uniform destination `lds`.
+__device__ __forceinline__ void copy16_to_lds(const uint32_t *src,
+                                              uint32_t *lds) {
+  __builtin_amdgcn_global_load_lds(src, lds, 16, 0, 0);
+}
+// Wait until this wave's global loads, LDS copies included, have landed.
+__device__ __forceinline__ void wait_global_loads() {
+  __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0), other counters unchecked
+}
+
+// One K-tile stage: the A rows (xa) and B rows (xb) of every term.  The two
+// stages of a kernel are separate __shared__ objects, so that the compiler
+// sees that loads in flight into one do not alias reads from the other.
+template <int NT, int TILE> struct alignas(16) SplitStage {
+  uint32_t xa[NT][TILE * kSK];
+  uint32_t xb[NT][TILE * kSK];
 };
+
+// [x lanes 0-31 | y lanes 0-31] and [x lanes 32-63 | y lanes 32-63] of each
+// register: the lane-half exchange of v_permlane32_swap.
+__device__ __forceinline__ u32x4 halves_lo(u32x4 x, u32x4 y) {
+  u32x4 r;
+#pragma unroll
+  for (int d = 0; d < 4; ++d)
+    r[d] = __builtin_amdgcn_permlane32_swap(x[d], y[d], false, false)[0];
+  return r;
+}
+__device__ __forceinline__ u32x4 halves_hi(u32x4 x, u32x4 y) {
+  u32x4 r;
+#pragma unroll
+  for (int d = 0; d < 4; ++d)
+    r[d] = __builtin_amdgcn_permlane32_swap(x[d], y[d], false, false)[1];
+  return r;
+}
 
 // Waves compute 64x64 sub-tiles as 2x2 blocks of 32x32x16 MFMA, whose
 // K = 16 reals are 8 from lanes 0-31 and 8 from lanes 32-63.
@@ -352,22 +399,32 @@ template <int NT, int TILE> struct alignas(16) SplitLds {
 // large one: a lane half h in {0, 1} supplies its term (a_h for A, b_h for B)
 // over the same 4 complex columns, so one MFMA adds t_a0 t_b0 + t_a1 t_b1
 // before its single FP32 rounding into the accumulator.
-//   (2 terms, 3 products): hi*hi over 8 columns, then [hi|lo] x [lo|hi]
+//   (2 terms, 3 products): [hi|lo] x [lo|hi] per 4 columns, and hi*hi over 8
+//     columns, whose operands are the lane-half exchange of two [hi|lo]
+//     (A) and [lo|hi] (B) fragments already in registers
 //   (2 terms, 4 products): [hi|lo] x [hi|lo] and [hi|lo] x [lo|hi]
 //   (3 terms, 6 products): [h|m] x [h|m], [h|m] x [m|h], [h|l] x [l|h]
 // Each costs the same number of MFMAs as issuing the products separately.
+// The imaginary-part operand Y = (im, -re) of a B fragment is formed in
+// registers.
 //
-// One K tile per LDS stage, single-buffered: the next tile's global loads are
-// in flight in registers while the current tile is consumed.  The column
-// panel is stored twice, as X and as Y = (im, -re), so the imaginary-part
-// MFMAs read their operand instead of forming it per use.  Threads that
-// stage a diagonal tile's rows also sum |t0 + t1 (+ t2)|^2 of them in FP64.
+// K tiles go through two LDS stages.  Each wave copies its share of the next
+// tile's rows straight from global memory into the other stage
+// (global_load_lds_dwordx4: 16 bytes a lane, lane l to byte 16 l of a 1 KB
+// run of 16 rows), then computes the current tile; one barrier per K tile
+// publishes the copy and frees the stage just read.  Rows past m are read as
+// row m - 1: they only reach output rows and columns past m, which nothing
+// reads.  For a diagonal tile the threads also sum |t0 + t1 (+ t2)|^2 of the
+// staged rows in FP64 (the exact diagonal).
 template <int NT, int MAXORD, bool F16, int TILE>
 __device__ __forceinline__ void split_item(const SplitProblem &P, int slice,
                                            int tile, int ns, int ktiles,
                                            int kps, int64_t seg_len,
-                                           SplitLds<NT, TILE> &L) {
+                                           SplitStage<NT, TILE> &L0,
+                                           SplitStage<NT, TILE> &L1) {
   typedef SplitShape<TILE> S;
+  constexpr int kWaves = S::kThreads / 64;
+  constexpr int kRuns = TILE / 16 / kWaves;  // 16-row runs a wave copies
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = TILE == 128 ? wave >> 1 : 0, wc = TILE == 128 ? wave & 1 : 0;
   const int lr = lane & 31, lh = lane >> 5;
@@ -378,20 +435,27 @@ __device__ __forceinline__ void split_item(const SplitProblem &P, int slice,
                       bj * TILE + wc * 64 < P.m;
   const int kt0 = (int)((int64_t)slice * ktiles / ns);
   const int kt1 = (int)((int64_t)(slice + 1) * ktiles / ns);
+  // copy lanes: run g = wave * kRuns + u covers rows 16 g .. 16 g + 15;
+  // lane l fills LDS chunk l % 4 of row 16 g + l / 4, i.e. logical chunk
+  // (l % 4) ^ swz(row) of that row
+  int64_t coff_a[kRuns], coff_b[kRuns];
+#pragma unroll
+  for (int u = 0; u < kRuns; ++u) {
+    const int r = (wave * kRuns + u) * 16 + (lane >> 2);
+    const int c = (lane & 3) ^ swz(r);
+    const int ra = bi * TILE + r, rb = bj * TILE + r;
+    const int ga = ra < P.m ? ra : P.m - 1, gb = rb < P.m ? rb : P.m - 1;
+    coff_a[u] = (int64_t)ga * P.ldz + 4 * c;
+    coff_b[u] = (int64_t)gb * P.ldz + 4 * c;
+  }
+  // diagonal sums: piece pi covers half lw of row lrow (8 words)
   int lrow[S::kPairs], lw[S::kPairs];
-  bool ok_a[S::kPairs], ok_b[S::kPairs];
-  int64_t roff_a[S::kPairs], roff_b[S::kPairs];
   double dsum[S::kPairs];
 #pragma unroll
   for (int p = 0; p < S::kPairs; ++p) {
-    const int pi = tid + p * S::kThreads;
-    lrow[p] = pi >> 1;
-    lw[p] = (pi & 1) * 8;
-    const int ga = bi * TILE + lrow[p], gb = bj * TILE + lrow[p];
-    ok_a[p] = ga < P.m;
-    ok_b[p] = gb < P.m;
-    roff_a[p] = (int64_t)ga * P.ldz + lw[p];
-    roff_b[p] = (int64_t)gb * P.ldz + lw[p];
+    const int pi = tid + p * S::kThreads, l = pi & 63;
+    lrow[p] = (pi >> 6) * 32 + 16 * (l >> 5) + (l & 15);
+    lw[p] = ((l >> 4) & 1) * 8;
     dsum[p] = 0.0;
   }
 
@@ -404,136 +468,139 @@ __device__ __forceinline__ void split_item(const SplitProblem &P, int slice,
       acc_im[i][j] = v16f{};
     }
 
-  // position of the K tile to load next: segment sg, tile kc within it
-  int sg = kt0 / (kps > 0 ? kps : 1), kc = kt0 - sg * kps;
-  u32x4 ra[S::kPairs][NT][2], rb[S::kPairs][NT][2];
+  // position of the K tile to copy next: segment sg, tile kc within it;
   // whole K tiles: columns [seg_len, round_up(seg_len, 16)) of every
-  // segment are zero by contract, rows >= m are not read
-  auto gload1 = [&](int64_t roff, bool row_ok, u32x4 (*r)[2]) {
-    const int64_t off = roff + sg * P.seg_stride + kc * kSK;
+  // segment are zero by contract
+  int sg = kt0 / (kps > 0 ? kps : 1), kc = kt0 - sg * kps;
+  auto copy = [&](SplitStage<NT, TILE> &D) {
+    const int64_t k0 = sg * P.seg_stride + kc * kSK;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const uint32_t *src = P.t[t] + off;
-      if (row_ok) {
-        r[t][0] = *(const u32x4 *)src;
-        r[t][1] = *(const u32x4 *)(src + 4);
-      } else {
-        r[t][0] = u32x4{0u, 0u, 0u, 0u};
-        r[t][1] = u32x4{0u, 0u, 0u, 0u};
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int u = 0; u < kRuns; ++u) {
+        const int run = (wave * kRuns + u) * 16 * kSK;
+        copy16_to_lds(P.t[t] + k0 + coff_a[u], &D.xa[t][run]);
+        if (!diag) copy16_to_lds(P.t[t] + k0 + coff_b[u], &D.xb[t][run]);
       }
-    }
-  };
-  auto gload = [&]() {
-#pragma unroll
-    for (int p = 0; p < S::kPairs; ++p) {
-      gload1(roff_a[p], ok_a[p], ra[p]);
-      if (!diag) gload1(roff_b[p], ok_b[p], rb[p]);
-    }
     if (++kc == kps) {
       kc = 0;
       ++sg;
     }
   };
-
-  if (kt0 < kt1) gload();
-  for (int kt = kt0; kt < kt1; ++kt) {
-    __syncthreads();  // every wave is done reading the previous tile
+  auto diag_sums = [&](const SplitStage<NT, TILE> &D) {
 #pragma unroll
     for (int p = 0; p < S::kPairs; ++p) {
-      const int o = lrow[p] * kSRow + lw[p];
+      u32x4 r[NT][2];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        *(u32x4 *)&L.xa[t][o] = ra[p][t][0];
-        *(u32x4 *)&L.xa[t][o + 4] = ra[p][t][1];
-        if (diag) {
-          *(u32x4 *)&L.yb[t][o] = swap_negate4(ra[p][t][0]);
-          *(u32x4 *)&L.yb[t][o + 4] = swap_negate4(ra[p][t][1]);
-          if (t == NT - 1) dsum[p] += row_norm2<NT, F16>(ra[p]);
-        } else {
-          *(u32x4 *)&L.xb[t][o] = rb[p][t][0];
-          *(u32x4 *)&L.xb[t][o + 4] = rb[p][t][1];
-          *(u32x4 *)&L.yb[t][o] = swap_negate4(rb[p][t][0]);
-          *(u32x4 *)&L.yb[t][o + 4] = swap_negate4(rb[p][t][1]);
-        }
+        r[t][0] = *(const u32x4 *)&D.xa[t][lds_word(lrow[p], lw[p])];
+        r[t][1] = *(const u32x4 *)&D.xa[t][lds_word(lrow[p], lw[p] + 4)];
       }
+      dsum[p] += row_norm2<NT, F16>(r);
     }
-    __syncthreads();
-    if (kt + 1 < kt1) gload();
-    if (!active) continue;
-    const uint32_t(*Bx)[TILE * kSRow] = diag ? L.xa : L.xb;
-    // operands of the 2x2 blocks: lane half lh reads term t_lh at word
-    // offset w_lh of its row
+  };
+  // the MFMAs of one K tile from stage D
+  auto compute = [&](const SplitStage<NT, TILE> &D) {
+    const uint32_t(*Ax)[TILE * kSK] = D.xa;
+    const uint32_t(*Bx)[TILE * kSK] = diag ? D.xa : D.xb;
+    // lane half lh reads term t_lh at word offset w_lh of its row
     auto ld_a = [&](int t0, int t1, int w0, int w1, u32x4 (&a)[2]) {
-      const uint32_t *base = L.xa[lh ? t1 : t0];
+      const uint32_t *base = Ax[lh ? t1 : t0];
       const int w = lh ? w1 : w0;
 #pragma unroll
       for (int i = 0; i < 2; ++i)
-        a[i] = *(const u32x4 *)&base[(wr * 64 + i * 32 + lr) * kSRow + w];
+        a[i] = *(const u32x4 *)&base[lds_word(wr * 64 + i * 32 + lr, w)];
     };
-    auto ld_b = [&](int t0, int t1, int w0, int w1, u32x4 (&b)[2],
-                    u32x4 (&y)[2]) {
-      const int t = lh ? t1 : t0, w = lh ? w1 : w0;
+    auto ld_b = [&](int t0, int t1, int w0, int w1, u32x4 (&b)[2]) {
+      const uint32_t *base = Bx[lh ? t1 : t0];
+      const int w = lh ? w1 : w0;
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        b[j] = *(const u32x4 *)&base[lds_word(wc * 64 + j * 32 + lr, w)];
+    };
+    auto mma = [&](const u32x4 (&a)[2], const u32x4 (&b)[2]) {
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        const int o = (wc * 64 + j * 32 + lr) * kSRow + w;
-        b[j] = *(const u32x4 *)&Bx[t][o];
-        y[j] = *(const u32x4 *)&L.yb[t][o];
+        const u32x4 y = swap_negate4(b[j]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          acc_re[i][j] = SplitMfma<F16>::op(a[i], b[j], acc_re[i][j]);
+          acc_im[i][j] = SplitMfma<F16>::op(a[i], y, acc_im[i][j]);
+        }
       }
     };
-    auto mma = [&](const u32x4 (&a)[2], const u32x4 (&b)[2],
-                   const u32x4 (&y)[2]) {
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          acc_re[i][j] = SplitMfma<F16>::op(a[i], b[j], acc_re[i][j]);
-          acc_im[i][j] = SplitMfma<F16>::op(a[i], y[j], acc_im[i][j]);
-        }
-    };
-    u32x4 a[2], b[2], y[2];
+    u32x4 a[2], b[2];
     if constexpr (NT == 2 && MAXORD == 1) {
 #pragma unroll
       for (int ks = 0; ks < 2; ++ks) {
-        ld_a(0, 0, 8 * ks, 8 * ks + 4, a);
-        ld_b(0, 0, 8 * ks, 8 * ks + 4, b, y);
-        mma(a, b, y);
-      }
+        u32x4 a1[2], b1[2];
+        ld_a(0, 1, 8 * ks, 8 * ks, a);  // [hi|lo] of columns 8ks..8ks+3
+        ld_b(1, 0, 8 * ks, 8 * ks, b);  // [lo|hi]
+        mma(a, b);
+        ld_a(0, 1, 8 * ks + 4, 8 * ks + 4, a1);
+        ld_b(1, 0, 8 * ks + 4, 8 * ks + 4, b1);
+        mma(a1, b1);
+        // hi of columns 8ks..8ks+3 in lanes 0-31 and of 8ks+4..8ks+7 in
+        // lanes 32-63, for A and B alike
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        ld_a(0, 1, 4 * q, 4 * q, a);
-        ld_b(1, 0, 4 * q, 4 * q, b, y);
-        mma(a, b, y);
+        for (int i = 0; i < 2; ++i) {
+          a[i] = halves_lo(a[i], a1[i]);
+          b[i] = halves_hi(b[i], b1[i]);
+        }
+        mma(a, b);
       }
     } else if constexpr (NT == 2) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         ld_a(0, 1, 4 * q, 4 * q, a);
-        ld_b(0, 1, 4 * q, 4 * q, b, y);
-        mma(a, b, y);
-        ld_b(1, 0, 4 * q, 4 * q, b, y);
-        mma(a, b, y);
+        ld_b(0, 1, 4 * q, 4 * q, b);
+        mma(a, b);
+        ld_b(1, 0, 4 * q, 4 * q, b);
+        mma(a, b);
       }
     } else {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         ld_a(0, 1, 4 * q, 4 * q, a);
-        ld_b(0, 1, 4 * q, 4 * q, b, y);
-        mma(a, b, y);
-        ld_b(1, 0, 4 * q, 4 * q, b, y);
-        mma(a, b, y);
+        ld_b(0, 1, 4 * q, 4 * q, b);
+        mma(a, b);
+        ld_b(1, 0, 4 * q, 4 * q, b);
+        mma(a, b);
         ld_a(0, 2, 4 * q, 4 * q, a);
-        ld_b(2, 0, 4 * q, 4 * q, b, y);
-        mma(a, b, y);
+        ld_b(2, 0, 4 * q, 4 * q, b);
+        mma(a, b);
       }
     }
+  };
+  // one K tile computed from stage C while the next is copied into N
+  auto step = [&](SplitStage<NT, TILE> &C, SplitStage<NT, TILE> &N, int kt) {
+    if (kt + 1 < kt1) copy(N);
+    if (diag) diag_sums(C);
+    if (active) {
+      // issue priority to waves in their MFMA phase over waves copying or
+      // waiting at the barrier
+      __builtin_amdgcn_s_setprio(1);
+      compute(C);
+      __builtin_amdgcn_s_setprio(0);
+    }
+    wait_global_loads();  // this wave's copies into N
+    lds_barrier();
+  };
+
+  if (kt0 < kt1) copy(L0);
+  wait_global_loads();
+  lds_barrier();
+  for (int kt = kt0; kt < kt1; kt += 2) {
+    step(L0, L1, kt);
+    if (kt + 1 < kt1) step(L1, L0, kt + 1);
   }
-  __syncthreads();  // LDS is reused by the next item
+  // every wave is past the last barrier: LDS is free for the next item
   if (diag) {
-    // two threads share each row; lanes tid and tid ^ 1 are in one wave
+    // two threads share each row: lanes l and l ^ 16 of one wave
 #pragma unroll
     for (int p = 0; p < S::kPairs; ++p) {
-      const double s = dsum[p] + __shfl_xor(dsum[p], 1, 64);
-      if ((tid & 1) == 0)
+      const double s = dsum[p] + __shfl_xor(dsum[p], 16, 64);
+      if (lw[p] == 0)
         P.dpart[((int64_t)slice * P.ntiles + tile) * TILE + lrow[p]] = s;
     }
   }
@@ -564,7 +631,7 @@ template <int NT, int MAXORD, bool F16, int TILE>
 __global__ __launch_bounds__(SplitShape<TILE>::kThreads,
                              TILE == 128 ? 2 : 8) void herk_split_kernel(
     SplitProblem P, SliceCfg cfg) {
-  __shared__ SplitLds<NT, TILE> L;
+  __shared__ SplitStage<NT, TILE> L0, L1;
   const int64_t seg_len = seg_columns(P.seg_len_host, P.seg_len_dev);
   const int kps = (int)cdiv(seg_len, (int64_t)kSK);  // K tiles per segment
   const int ktiles = kps * P.nseg;
@@ -572,7 +639,7 @@ __global__ __launch_bounds__(SplitShape<TILE>::kThreads,
   for_items((int64_t)P.ntiles * ns, [&](int64_t it) {
     split_item<NT, MAXORD, F16, TILE>(P, (int)(it / P.ntiles),
                                       (int)(it % P.ntiles), ns, ktiles, kps,
-                                      seg_len, L);
+                                      seg_len, L0, L1);
   });
 }
 
@@ -658,7 +725,7 @@ template <int NT, int MAXORD, bool F16, int TILE>
 __global__ __launch_bounds__(SplitShape<TILE>::kThreads,
                              TILE == 128 ? 2 : 8) void herk_split_grouped_kernel(
     const SplitProblem *__restrict__ probs, int nprob, GroupPlan plan) {
-  __shared__ SplitLds<NT, TILE> L;
+  __shared__ SplitStage<NT, TILE> L0, L1;
   const int64_t items = plan.item_off[nprob];
   for_items(items, [&](int64_t it) {
     const int p = find_segment(plan.item_off, nprob, it);
@@ -669,7 +736,7 @@ __global__ __launch_bounds__(SplitShape<TILE>::kThreads,
     const int kps = (int)cdiv(seg_len, (int64_t)kSK);
     split_item<NT, MAXORD, F16, TILE>(P, (int)(local / P.ntiles),
                                       (int)(local % P.ntiles), ns,
-                                      kps * P.nseg, kps, seg_len, L);
+                                      kps * P.nseg, kps, seg_len, L0, L1);
   });
 }
 

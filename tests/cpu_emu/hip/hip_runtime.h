@@ -62,6 +62,10 @@ bool emu_lane_live(int lane);
 void emu_wave_barrier();
 
 inline void __syncthreads() { emu_syncthreads(); }
+// LDS-only fences and the bare barrier: the emulated waves see memory in
+// program order, so the fences are no-ops and the barrier is the block's.
+#define __builtin_amdgcn_fence(order, scope, ...) ((void)0)
+#define __builtin_amdgcn_s_barrier() emu_syncthreads()
 inline unsigned long long __ballot(int p) {
   *(int *)emu_wave_slot(emu_lane()) = p ? 1 : 0;
   emu_wave_barrier();
@@ -93,6 +97,43 @@ template <class T> inline T __shfl_xor(T v, int mask, int width = 64) {
   emu_wave_barrier();
   return r;
 }
+// v_permlane32_swap as measured on gfx950: result [0] is x of lanes 0-31 in
+// lanes 0-31 and y of lanes 0-31 in lanes 32-63; result [1] is x of lanes
+// 32-63 in lanes 0-31 and y of lanes 32-63 in lanes 32-63.
+struct emu_u32x2 {
+  unsigned v[2];
+  unsigned operator[](int i) const { return v[i]; }
+};
+inline emu_u32x2 emu_permlane32_swap(unsigned x, unsigned y) {
+  const int l = emu_lane();
+  const unsigned me[2] = {x, y};
+  memcpy(emu_wave_slot(l), me, sizeof me);
+  emu_wave_barrier();
+  unsigned other[2];
+  emu_u32x2 r;
+  if (l < 32) {
+    memcpy(other, emu_wave_slot(l + 32), sizeof other);
+    r.v[0] = x;
+    r.v[1] = other[0];
+  } else {
+    memcpy(other, emu_wave_slot(l - 32), sizeof other);
+    r.v[0] = other[1];
+    r.v[1] = y;
+  }
+  emu_wave_barrier();
+  return r;
+}
+#define __builtin_amdgcn_permlane32_swap(x, y, fi, bc) emu_permlane32_swap(x, y)
+// global_load_lds: lane l copies `size` bytes from its source to byte
+// off + size * l of the wave's LDS destination.  Waves see memory in program
+// order here, so waits on the copies are no-ops.
+inline void emu_global_load_lds(const void *src, void *dst, unsigned size) {
+  memcpy((char *)dst + (size_t)emu_lane() * size, src, size);
+}
+#define __builtin_amdgcn_global_load_lds(src, dst, size, off, aux) \
+  emu_global_load_lds(src, (char *)(dst) + (off), size)
+#define __builtin_amdgcn_s_waitcnt(imm) ((void)0)
+#define __builtin_amdgcn_s_setprio(p) ((void)0)
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline unsigned __float_as_uint(float f) {
   unsigned u;
