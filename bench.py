@@ -276,6 +276,90 @@ def batched_bench(ninst: int = 256, seed: int = 7, reps: int = 5):
     }
 
 
+def diffuse_bench(ntimes: int = 2, warmup: int = 1):
+    """Diffuse-sky scale: HEALPix nside 128 (196,608 pixel centres),
+    log-normal(-2, 1.5) brightness, 350 hex antennas, polarized, four +-5%
+    perturbations of the 360 x 91 E-field beam assigned round-robin
+    (beam_idx = ant % 4), 8 frequencies over 120-180 MHz, source chunks of
+    16,384 (12 chunks, FP64 cross-chunk sums).  Times `ntimes` time samples
+    of the plain FP64 and the split-BF16 forms; errors of split-BF16 against
+    FP64 on the first time, and of an FP32-only cross-chunk sum (each
+    chunk's FP64 visibilities rounded to FP32 and summed in FP32) at the
+    first frequency."""
+    import torch
+
+    from matvis_amd import astro, synth
+    from matvis_amd.gpu.gpu import build_engine, simulate
+    from matvis_amd.metrics import normalized_error
+
+    ra, dec = synth.healpix_centres(128)
+    flux0 = synth.lognormal_fluxes(ra.size, 3, -2.0, 1.5)
+    nant, nfreq = 350, 8
+    antpos = synth.hex_array(nant, 14.6, 0.02, seed=3)
+    beams = [synth.harmonic_efield_beam(seed=4, naz=360, nza=91, perturb=0.05,
+                                        perturb_seed=100 + b)
+             for b in range(4)]
+    freqs = np.linspace(120e6, 180e6, nfreq)
+    times = synth.times_spaced(16, 10.0)
+    loc = synth.hera_location()
+    rot = astro.rotation_matrices(times, loc)
+    common = dict(antpos=antpos, times=times, skycoords=astro.SkyCoord(ra, dec),
+                  telescope_loc=loc, beam_list=beams,
+                  beam_idx=np.arange(nant) % 4, polarized=True, precision=2,
+                  coord_method="CoordinateRotationMatrix",
+                  coord_method_params={"rotations": rot},
+                  beam_spline_opts={"order": 3}, min_chunks=12)
+    fluxes = np.repeat(flux0[:, None], nfreq, 1)
+    out = {"nsrc": int(ra.size), "nant": nant, "nfreq": nfreq, "chunks": 12,
+           "chunk_sources": 16384, "times_timed": ntimes}
+    first = {}
+    vis_per_time = nfreq * nant * nant * 4
+    for name, method in (("fp64", "GPUMatMul"),
+                         ("bf16", "GPUMatMulSplitBF16")):
+        log(f"diffuse: {name} engine")
+        eng, _ = build_engine(freqs=freqs, fluxes=fluxes,
+                              matprod_method=method, **common)
+        dev = eng.new_output()
+        stream = torch.cuda.current_stream()
+        for t in range(warmup):
+            eng.compute_time(t, dev, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t in range(ntimes):
+            eng.compute_time(warmup + t, dev, stream=stream)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        eng.compute_time(0, dev, stream=stream)
+        first[name] = dev.cpu().numpy()
+        out[name] = {"ms_per_time": 1e3 * sec / ntimes,
+                     "vis_per_s": vis_per_time * ntimes / sec,
+                     "nchunks": eng.nchunks}
+        del eng, dev
+        torch.cuda.empty_cache()
+    e = [normalized_error(first["bf16"][f], first["fp64"][f], nant, 2)
+         for f in range(nfreq)]
+    out["bf16_vs_fp64_max_median"] = [max(x[0] for x in e),
+                                      float(np.median([x[1] for x in e]))]
+    # FP32-only cross-chunk sums at the first frequency and time
+    log("diffuse: FP32-only chunk sums")
+    one = dict(common, times=times[:1], coord_method_params={"rotations": rot[:1]},
+               min_chunks=1)
+    acc32 = None
+    n = ra.size
+    for c in range(12):
+        sl = slice(c * n // 12, (c + 1) * n // 12)
+        v = simulate(freq=float(freqs[0]), I_sky=flux0[sl],
+                     **dict(one, skycoords=astro.SkyCoord(ra[sl], dec[sl])))
+        v = v.astype(np.complex64)
+        acc32 = v if acc32 is None else acc32 + v
+    ref = first["fp64"][0]
+    out["fp32_chunk_sums_vs_fp64_max_median"] = list(normalized_error(
+        acc32[0].astype(np.complex128), ref, nant, 2)[:2])
+    out["speedup_bf16_over_fp64"] = (out["bf16"]["vis_per_s"]
+                                     / out["fp64"]["vis_per_s"])
+    return out
+
+
 SERVE_NANTS = (32, 128, 350, 512)
 
 
@@ -529,6 +613,8 @@ def main():
                     help="mean requests per second of the serving run")
     ap.add_argument("--serve-drain", type=float, default=20.0,
                     help="seconds after the window for requests to finish")
+    ap.add_argument("--no-diffuse", action="store_true",
+                    help="skip the diffuse-sky (HEALPix nside 128) run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy first (the "
                          "known-size kernel that calibrates HBM byte counters)")
@@ -674,6 +760,8 @@ def main():
     if not args.no_batched:
         log("batched: grouped HERK over 256 mixed instances")
         line["batched"] = batched_bench()
+    if not args.no_diffuse:
+        line["diffuse"] = diffuse_bench()
     if not args.no_serving:
         line["serving"] = serving_bench(args.serve_seconds, args.serve_rate,
                                         args.serve_drain)
