@@ -150,6 +150,36 @@ def test_split_forms_match_oracle(method, terms, prods, bound):
     assert e[0] < bound
 
 
+@pytest.mark.parametrize("precision,method,bound", [
+    (2, "GPUMatMul", 1e-12), (1, "GPUMatMul", 1e-5),
+    (2, "GPUMatMulSplitBF16", 1e-5)])
+def test_cpu_runnable_configuration(precision, method, bound):
+    """The CPU-runnable configuration through simulate_vis: 16 antennas in a
+    100 m disk, 2,000 sources (RA uniform, sin(dec) uniform on [-90, +30]
+    deg, power-law S^-2 fluxes on [0.1, 10] Jy), 150 MHz, 10 times 6 min
+    apart, Gaussian beam sigma = 10 deg, unpolarized; output
+    (1, ntimes, nant^2) against the oracle.  The split bound is wider than
+    the 100k-source sky's: a power-law sky's few bright sources do not
+    average the bf16 representation residual down."""
+    import matvis_amd
+
+    p = small_problem(nant=16, nsrc=2000, ntimes=10, polarized=False,
+                      beam="gaussian")
+    ants = {i: tuple(p["antpos"][i]) for i in range(16)}
+    vis = matvis_amd.simulate_vis(
+        ants, p["I_sky"][:, None], p["ra"], p["dec"], np.array([p["freq"]]),
+        p["times"], p["beam_list"], p["loc"], polarized=False,
+        precision=precision, use_gpu=True, matprod_method=method,
+        coord_method="CoordinateRotationMatrix",
+        coord_method_params={"rotations": p["rotations"]})
+    assert vis.shape == (1, 10, 256)
+    assert vis.dtype == (np.complex128 if precision == 2 else np.complex64)
+    e = nerr(vis[0], run_oracle(p), 16, False)
+    print(f"16 ant x 2000 src x 10 times, {method} precision {precision}: "
+          f"normalised max {e[0]:.2e} median {e[1]:.2e}", flush=True)
+    assert e[0] < bound
+
+
 def test_simulate_vis_frequencies_match_single_calls():
     """wrapper.py:180-199: the frequency loop; here rotations are shared."""
     import matvis_amd
