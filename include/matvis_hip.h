@@ -86,6 +86,20 @@ int mv_rotate_cull(const double *eci, int64_t ld_eci, const double *rot9,
                    int32_t *n_up_dev, void *workspace, size_t ws_bytes,
                    mv_stream_t stream);
 
+/* BCRS corrections of ICRS unit vectors, the per-source part of the ERFA
+ * rotation: light deflection by the Sun (limiter dlim, 1e-6 in the
+ * reference), aberration and the bias-precession-nutation matrix, giving CIRS
+ * proper directions.  Replaces CoordinateRotationERFA._set_bcrs with _ld,
+ * _ab and _bpn (src/matvis/cpu/coords.py:110-215) and the light-deflection
+ * kernel of GPUCoordinateRotationERFA (src/matvis/gpu/coords.py:7-53).
+ * eci, out: (3, ld) f64 SoA, nsrc sources (out may alias eci).
+ * ctx17 (host): eh[3] (Sun -> observer unit vector), em (au), v[3]
+ * (barycentric velocity / c), bm1 (sqrt(1 - |v|^2)), bpn[9] row-major.
+ * mv_rotate_cull then applies the per-time CIRS -> ENU matrix to `out`. */
+int mv_bcrs_update(const double *eci, int64_t ld_eci, int64_t nsrc,
+                   const double *ctx17, double dlim, double *out,
+                   int64_t ld_out, mv_stream_t stream);
+
 /* ---------------------------------------------------------------- (b) ---- */
 
 /* In-place cubic B-spline prefilter of beam planes (mirror boundary), the

@@ -14,6 +14,17 @@ deflection, polar motion and UT1-UTC, so it agrees with astropy's AltAz to
 about 20-30 arcsec, not to the 10 mas of the reference's ERFA path.  The same
 R_t feeds the device kernels and the host oracle, so parity of the per-time
 computation does not depend on this accuracy.
+
+:func:`erfa_context` gives the per-time astrometry context of the ERFA-style
+rotation (the fields of ERFA's ``eraApco13`` that matvis's ERFA path uses):
+the observer's heliocentric direction and distance and barycentric velocity
+from the low-precision solar series of the Astronomical Almanac
+(differentiated for the velocity; the Sun's motion about the barycentre and
+the Moon's pull on the Earth are omitted, ~1e-3 of the velocity) plus the
+observer's rotational velocity, the Earth rotation angle (IAU 2000, UT1 =
+UTC) and a bias-precession-nutation matrix built from the model above so
+that the context reproduces R_t exactly when light deflection and
+aberration are left out.
 """
 
 from __future__ import annotations
@@ -216,3 +227,84 @@ def rotation_matrices(times, telescope_loc) -> np.ndarray:
     return np.stack(
         [eci_to_enu_rotation(jd, lat, lon) for jd in times_jd(times)]
     )
+
+
+# ------------------------------------------------------------------ ERFA-style context
+
+#: speed of light in au per day (IAU 2012 au, c = 299792458 m/s)
+C_AU_PER_DAY = 299792458.0 * 86400.0 / 149597870700.0
+#: Earth rotation angle rate, rad per UT1 day (IAU 2000)
+ERA_RATE = 2.0 * math.pi * 1.00273781191135448
+WGS84_A, WGS84_F = 6378137.0, 1.0 / 298.257223563
+
+
+def earth_rotation_angle(jd_ut1: float) -> float:
+    """IAU 2000 Earth rotation angle (rad)."""
+    d = jd_ut1 - J2000
+    f = (jd_ut1 % 1.0) + (J2000 % 1.0)  # fraction of day, split for precision
+    return (2.0 * math.pi * (f + 0.7790572732640 + 0.00273781191135448 * d)) \
+        % (2.0 * math.pi)
+
+
+def sun_geocentric(jd_tt: float):
+    """Geocentric position (au) and velocity (au/day) of the Sun on the mean
+    equator and equinox of date, from the Astronomical Almanac's
+    low-precision series (about 0.01 deg in longitude, 1950-2050):
+    L = 280.460 + 0.9856474 n, g = 357.528 + 0.9856003 n (deg),
+    lambda = L + 1.915 sin g + 0.020 sin 2g (+20.5" to undo the annual
+    aberration folded into L), R = 1.00014 - 0.01671 cos g - 0.00014 cos 2g,
+    obliquity 23.439 - 4e-7 n; n = days from J2000."""
+    n = jd_tt - J2000
+    deg = math.pi / 180.0
+    L = (280.460 + 0.9856474 * n) * deg
+    g = (357.528 + 0.9856003 * n) * deg
+    dg = 0.9856003 * deg  # rad/day
+    lam = L + (1.915 * math.sin(g) + 0.020 * math.sin(2 * g)) * deg \
+        + 20.496 * ARCSEC
+    dlam = 0.9856474 * deg + (1.915 * math.cos(g) + 0.040 * math.cos(2 * g)) \
+        * deg * dg
+    r = 1.00014 - 0.01671 * math.cos(g) - 0.00014 * math.cos(2 * g)
+    dr = (0.01671 * math.sin(g) + 0.00028 * math.sin(2 * g)) * dg
+    eps = (23.439 - 0.0000004 * n) * deg
+    ce, se = math.cos(eps), math.sin(eps)
+    u = np.array([math.cos(lam), ce * math.sin(lam), se * math.sin(lam)])
+    du = np.array([-math.sin(lam), ce * math.cos(lam), se * math.cos(lam)])
+    return r * u, dr * u + r * dlam * du
+
+
+def erfa_context(jd_utc: float, lat: float, lon: float,
+                 height: float = 0.0) -> dict:
+    """Astrometry context at one UTC Julian date for a geodetic site
+    (radians, metres): ``eh`` (unit vector Sun -> observer, GCRS), ``em``
+    (au), ``v`` (observer barycentric velocity / c, GCRS), ``bm1``,
+    ``bpn`` (GCRS -> the frame ``eral`` rotates), ``eral`` (local Earth
+    rotation angle), ``xpl = ypl = 0`` (no polar motion), ``sphi``,
+    ``cphi``.  UT1 = UTC."""
+    jd_tt = jd_utc + TT_MINUS_UTC_S / 86400.0
+    prec = precession_iau1976(jd_tt)
+    dpsi, deps, eps0 = nutation_iau1980_leading(jd_tt)
+    nut = _r1(-(eps0 + deps)) @ _r3(-dpsi) @ _r1(eps0)
+    gast = gmst_iau1982(jd_utc) + dpsi * math.cos(eps0 + deps)
+    era = earth_rotation_angle(jd_utc)
+    bpn = _r3(gast - era) @ nut @ prec  # R3(era + lon) bpn = R3(last) N P
+    eral = (era + lon) % (2.0 * math.pi)
+    # Earth about the Sun (mean equator of date -> GCRS by the precession)
+    s_pos, s_vel = sun_geocentric(jd_tt)
+    s_pos, s_vel = prec.T @ s_pos, prec.T @ s_vel
+    em = float(np.linalg.norm(s_pos))
+    eh = -s_pos / em
+    v = -s_vel / C_AU_PER_DAY
+    # the site's rotation about the Earth's axis, in the rotating frame of
+    # eral, back to GCRS
+    e2 = WGS84_F * (2.0 - WGS84_F)
+    nrad = WGS84_A / math.sqrt(1.0 - e2 * math.sin(lat) ** 2)
+    rxy = (nrad + height) * math.cos(lat)
+    omega = ERA_RATE / 86400.0  # rad/s
+    v_site = np.array([-rxy * math.sin(eral), rxy * math.cos(eral), 0.0]) \
+        * omega / 299792458.0
+    v = v + bpn.T @ v_site
+    return {
+        "eh": eh, "em": em, "v": v, "bm1": math.sqrt(1.0 - float(v @ v)),
+        "bpn": bpn, "eral": eral, "xpl": 0.0, "ypl": 0.0,
+        "sphi": math.sin(lat), "cphi": math.cos(lat),
+    }

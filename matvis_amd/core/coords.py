@@ -77,7 +77,10 @@ class CoordinateRotation(ABC):
         else:
             self.nsrc_alloc = self.chunk_size
         self._rot = None
-        self._mode = None  # "eci" (R_t on device) or "topo" (filled vectors)
+        # "eci" (R_t applied to the ICRS vectors), "topo" (host-filled ENU
+        # vectors, identity rotation) or "bcrs" (the CIRS directions of the
+        # ERFA-style path, with its per-time CIRS -> ENU matrix)
+        self._mode = None
         self._topo_valid = False
 
     # ----------------------------------------------------------- buffers
@@ -123,13 +126,21 @@ class CoordinateRotation(ABC):
         self._mode = "topo"
         self._topo_valid = False
 
+    def rotation_source(self):
+        """Device (3, nsrc) f64 vectors the current rotation applies to."""
+        if self._mode == "topo":
+            return self._topo64
+        if self._mode == "bcrs":
+            return self._bcrs
+        return self.eci
+
     @property
     def all_coords_topo(self):
         """Topocentric unit vectors of every source, (3, nsrc), rtype."""
         if self._mode is None:
             raise RuntimeError("call rotate(t) first")
         if not self._topo_valid:
-            src = self.eci if self._mode == "eci" else self._topo64
+            src = self.rotation_source()
             call("mv_rotate", D.ptr(src), self.nsrc, self._rot.ctypes.data,
                  self.nsrc, D.ptr(self._topo), int(self.precision == 2),
                  self.nsrc, D.stream_ptr())
@@ -143,7 +154,7 @@ class CoordinateRotation(ABC):
         stays on the device in ``n_up_dev``."""
         if self._mode is None:
             raise RuntimeError("call rotate(t) first")
-        src = self.eci if self._mode == "eci" else self._topo64
+        src = self.rotation_source()
         chunk0 = chunk * self.chunk_size
         chunk_len = max(0, min(self.chunk_size, self.nsrc - chunk0))
         chunk0 = min(chunk0, self.nsrc)

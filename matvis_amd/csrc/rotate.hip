@@ -194,6 +194,85 @@ extern "C" int mv_rotate(const double *eci, int64_t ld_eci, const double *rot9,
   return check_launch("mv_rotate");
 }
 
+namespace mv {
+namespace {
+
+// Per-time astrometry context of the BCRS corrections: observer direction
+// from the Sun and distance (au), barycentric velocity (units of c) and
+// sqrt(1 - |v|^2), bias-precession-nutation matrix.
+struct BcrsCtx {
+  double eh[3], em, v[3], bm1;
+  Mat3 bpn;
+};
+
+constexpr double kSunSchwarzschildAu = 1.97412574336e-8;
+
+// Light deflection by the Sun (limiter dlim), aberration, then the
+// bias-precession-nutation matrix, for one unit vector: ICRS -> CIRS proper
+// direction.  One source per thread, SoA in and out.
+__global__ __launch_bounds__(kThreads) void bcrs_kernel(
+    const double *__restrict__ eci, int64_t ld_eci, int64_t n, BcrsCtx c,
+    double dlim, double *__restrict__ out, int64_t ld_out) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  double p[3] = {eci[i], eci[ld_eci + i], eci[2 * ld_eci + i]};
+  // deflection: p + w p x (e x p), w = SRS / em / max(p . (p + e), dlim)
+  double qpe = p[0] * (p[0] + c.eh[0]) + p[1] * (p[1] + c.eh[1]) +
+               p[2] * (p[2] + c.eh[2]);
+  if (qpe < dlim) qpe = dlim;
+  const double w = kSunSchwarzschildAu / c.em / qpe;
+  const double ex[3] = {c.eh[1] * p[2] - c.eh[2] * p[1],
+                        c.eh[2] * p[0] - c.eh[0] * p[2],
+                        c.eh[0] * p[1] - c.eh[1] * p[0]};
+  const double d[3] = {p[1] * ex[2] - p[2] * ex[1], p[2] * ex[0] - p[0] * ex[2],
+                       p[0] * ex[1] - p[1] * ex[0]};
+  for (int k = 0; k < 3; ++k) p[k] += w * d[k];
+  // aberration: bm1 p + w1 v + w2 (v - (p.v) bm1 p), renormalized
+  const double pdv = p[0] * c.v[0] + p[1] * c.v[1] + p[2] * c.v[2];
+  const double w1 = 1.0 + pdv / (1.0 + c.bm1);
+  const double w2 = kSunSchwarzschildAu / c.em;
+  double q[3], r2 = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const double pb = c.bm1 * p[k];
+    q[k] = pb + w1 * c.v[k] + w2 * (c.v[k] - pdv * pb);
+    r2 += q[k] * q[k];
+  }
+  const double inv = 1.0 / sqrt(r2);
+  for (int k = 0; k < 3; ++k) q[k] *= inv;
+  for (int r = 0; r < 3; ++r)
+    out[r * ld_out + i] = __fma_rn(c.bpn.m[3 * r + 2], q[2],
+                                   __fma_rn(c.bpn.m[3 * r + 1], q[1],
+                                            c.bpn.m[3 * r] * q[0]));
+}
+
+}  // namespace
+}  // namespace mv
+
+extern "C" int mv_bcrs_update(const double *eci, int64_t ld_eci,
+                              int64_t nsrc, const double *ctx17, double dlim,
+                              double *out, int64_t ld_out,
+                              mv_stream_t stream) {
+  MV_REQUIRE(eci && ctx17 && out, "mv_bcrs_update: null pointer");
+  MV_REQUIRE(nsrc >= 0 && ld_eci >= nsrc && ld_out >= nsrc,
+             "mv_bcrs_update: bad sizes nsrc=%lld ld_eci=%lld ld_out=%lld",
+             (long long)nsrc, (long long)ld_eci, (long long)ld_out);
+  MV_REQUIRE(ctx17[3] > 0.0 && ctx17[7] > 0.0,
+             "mv_bcrs_update: context needs em > 0 and bm1 > 0");
+  if (nsrc == 0) return MV_OK;
+  BcrsCtx c;
+  for (int k = 0; k < 3; ++k) {
+    c.eh[k] = ctx17[k];
+    c.v[k] = ctx17[4 + k];
+  }
+  c.em = ctx17[3];
+  c.bm1 = ctx17[7];
+  c.bpn = load_mat(ctx17 + 8);
+  dim3 grid((unsigned)cdiv(nsrc, kThreads));
+  bcrs_kernel<<<grid, kThreads, 0, as_stream(stream)>>>(eci, ld_eci, nsrc, c,
+                                                        dlim, out, ld_out);
+  return check_launch("mv_bcrs_update");
+}
+
 extern "C" size_t mv_rotate_cull_workspace_bytes(int64_t chunk_len) {
   int64_t nb = chunk_len > 0 ? cdiv(chunk_len, kItemsPerBlock) : 1;
   return (size_t)(cdiv(nb * 4, 256) * 256);

@@ -5,35 +5,125 @@ and ``GPUCoordinateRotationERFA`` (cpu/coords.py:41-230, gpu/coords.py:38-53)
 and ``CoordinateRotationAstropy`` (cpu/coords.py:15-38).  Added here:
 ``CoordinateRotationMatrix`` for caller-supplied per-time matrices.
 
-The ERFA-named methods compute R_t on the host with the model of
-:mod:`matvis_amd.astro` (precession, leading nutation, apparent sidereal time);
-the device applies it fused with the horizon cull.  They do not yet apply the
-per-source BCRS corrections (light deflection, aberration) of the reference's
-``_set_bcrs``; ``update_bcrs_every`` is accepted for signature compatibility.
+The ERFA-named methods follow the reference's ERFA path: the per-source BCRS
+corrections (light deflection by the Sun, aberration, bias-precession-
+nutation; ``mv_bcrs_update`` on the device) are recomputed when more than
+``update_bcrs_every`` seconds have passed since the last evaluation, and each
+time applies its CIRS -> ENU matrix (Earth rotation angle, polar motion,
+HA/Dec -> ENU) fused with the horizon cull.  The per-time astrometry
+context comes from, in order: ``astrom`` (caller-supplied, one dict per
+time with the fields of ERFA's ``eraApco13``: eh, em, v, bm1, bpn, eral,
+xpl, ypl, sphi, cphi), astropy's ``erfa_astrom`` when the times are astropy
+``Time`` objects (what the reference calls, cpu/coords.py:165-166), else the
+host model :func:`matvis_amd.astro.erfa_context` (within 0.1 arcsec of
+ERFA's full chain; UT1 = UTC, no polar motion).
 """
 
 from __future__ import annotations
 
 import numpy as np
 
-from ..astro import rotation_matrices, skycoord_ra_dec, times_jd
+from .. import _device as D
+from .._lib import call
+from ..astro import (erfa_context, location_lat_lon, skycoord_ra_dec,
+                     times_jd)
 from ..core.coords import CoordinateRotation
 
 
+_CTX_FIELDS = ("eh", "em", "v", "bm1", "bpn", "eral", "xpl", "ypl", "sphi",
+               "cphi")
+
+
+def _astropy_context(time, location):
+    """The reference's context (cpu/coords.py:165-166) for astropy inputs,
+    else None."""
+    if type(time).__module__.split(".")[0] != "astropy":
+        return None
+    from astropy.coordinates import AltAz
+    from astropy.coordinates.erfa_astrom import erfa_astrom
+
+    astrom = erfa_astrom.get().apco(AltAz(obstime=time, location=location))
+    return {k: np.asarray(astrom[k], dtype=np.float64) for k in _CTX_FIELDS}
+
+
+def atioq_matrix(ctx: dict) -> np.ndarray:
+    """CIRS -> (east, north, up): Earth rotation angle, polar motion, HA/Dec
+    -> ENU (cpu/coords.py:78-108)."""
+    ce, se = np.cos(ctx["eral"]), np.sin(ctx["eral"])
+    c2h = np.array([[ce, se, 0.0], [-se, ce, 0.0], [0.0, 0.0, 1.0]])
+    sx, cx = np.sin(ctx["xpl"]), np.cos(ctx["xpl"])
+    sy, cy = np.sin(ctx["ypl"]), np.cos(ctx["ypl"])
+    pm = np.array([[cx, 0.0, sx], [sx * sy, cy, -cx * sy],
+                   [-sx * cy, sy, cx * cy]])
+    sphi, cphi = float(ctx["sphi"]), float(ctx["cphi"])
+    enu = np.array([[0.0, 1.0, 0.0], [-sphi, 0.0, cphi], [cphi, 0.0, sphi]])
+    return enu @ pm @ c2h
+
+
+def bcrs_context_array(ctx: dict) -> np.ndarray:
+    """The 17 doubles mv_bcrs_update takes: eh, em, v, bm1, bpn."""
+    return np.concatenate([np.asarray(ctx["eh"], np.float64).ravel(),
+                           [float(ctx["em"])],
+                           np.asarray(ctx["v"], np.float64).ravel(),
+                           [float(ctx["bm1"])],
+                           np.asarray(ctx["bpn"], np.float64).ravel()])
+
+
 class CoordinateRotationERFA(CoordinateRotation):
-    """Rigid per-time rotation R_t from the host astrometry model."""
+    """ICRS -> ENU per time as the reference's ERFA path (see the module
+    docstring).  ``update_bcrs_every``: seconds between re-evaluations of
+    the per-source corrections (0: every time); ``astrom``: optional
+    caller-supplied contexts, one per time."""
 
-    def __init__(self, update_bcrs_every=0.0, *args, **kwargs):
+    def __init__(self, update_bcrs_every=0.0, *args, astrom=None, **kwargs):
         super().__init__(*args, **kwargs)
-        self.update_bcrs_every = update_bcrs_every
-        self._rots = rotation_matrices(self.times, self.telescope_loc)
+        if hasattr(update_bcrs_every, "to_value"):  # an astropy Quantity
+            update_bcrs_every = update_bcrs_every.to_value("s")
+        self.update_bcrs_every = float(update_bcrs_every)
+        self._jd = times_jd(self.times)
+        if astrom is not None and len(astrom) != len(self._jd):
+            raise ValueError("astrom needs one context per time")
+        self._astrom = astrom
+        self._time_of_last_evaluation = None
 
-    def rotation_matrix(self, t: int) -> np.ndarray:
-        return self._rots[t]
+    def setup(self):
+        super().setup()
+        self._bcrs = D.zeros((3, self.nsrc), np.float64)
+        self._time_of_last_evaluation = None
+
+    def context(self, t: int) -> dict:
+        """Astrometry context of time index t."""
+        if self._astrom is not None:
+            return {k: np.asarray(self._astrom[t][k], dtype=np.float64)
+                    for k in _CTX_FIELDS}
+        ctx = _astropy_context(self.times[t], self.telescope_loc)
+        if ctx is not None:
+            return ctx
+        lat, lon = location_lat_lon(self.telescope_loc)
+        height = getattr(self.telescope_loc, "height", 0.0)
+        height = float(height.to_value("m") if hasattr(height, "to_value")
+                       else height)
+        return erfa_context(float(self._jd[t]), lat, lon, height)
+
+    def rotate(self, t: int) -> None:
+        """BCRS corrections when due (cpu/coords.py:186-215), then the
+        time's CIRS -> ENU matrix (cpu/coords.py:217-230)."""
+        ctx = self.context(t)
+        last = self._time_of_last_evaluation
+        if last is None or (self._jd[t] - self._jd[last]) * 86400.0 \
+                > self.update_bcrs_every:
+            call("mv_bcrs_update", D.ptr(self.eci), self.nsrc, self.nsrc,
+                 bcrs_context_array(ctx).ctypes.data, 1e-6, D.ptr(self._bcrs),
+                 self.nsrc, D.stream_ptr())
+            self._time_of_last_evaluation = t
+        self._rot = np.ascontiguousarray(atioq_matrix(ctx))
+        self._mode = "bcrs"
+        self._topo_valid = False
 
 
 class GPUCoordinateRotationERFA(CoordinateRotationERFA):
-    """Same rotation; the name the reference's GPU path registers."""
+    """The name the reference's GPU path registers (gpu/coords.py:38-53);
+    the corrections run on the device in both classes."""
 
     requires_gpu: bool = True
 

@@ -177,7 +177,7 @@ class VisibilityEngine:
             self.setup()
         co = self.coords
         co.rotate(t)
-        src = co.eci if co._mode == "eci" else co._topo64
+        src = co.rotation_source()
         rot = co._rot
         sp = D.stream_ptr(stream)
         nup = self.n_up[parity]

@@ -48,6 +48,11 @@ Parity status per function (see ``tests/test_oracle.py``):
                             test_cublas.py:13-25, explicit triple loop
 ``simulate``                pinned: anti-zenith source gives V == 0
                             (test_matvis_gpu.py:15-25); chunk-count invariance
+``erfa_ld``, ``erfa_ab``    pinned: pyerfa's eraLd / eraAb on the golden
+                            sources and contexts (tests/golden/erfa_vectors.npz)
+``erfa_bcrs``,              pinned: pyerfa's eraAtco13 observed directions; the
+``erfa_atioq_matrix``       residual equals the diurnal aberration the
+                            reference omits (checked by applying it)
 ==========================  =====================================================
 """
 
@@ -384,6 +389,58 @@ def simulate(
             z = zmatrix(fl, a, et, beam_idx, nfeed, nax)
             vis[t] += vis_from_z(z, nant, nfeed, antpairs)
     return vis if polarized else vis[:, :, 0, 0]
+
+
+#: Schwarzschild radius of the Sun in au (cpu/coords.py:10).
+ERFA_SRS = 1.97412574336e-8
+
+
+def erfa_ld(p, e, em: float, dlim: float):
+    """Light deflection by the Sun (cpu/coords.py:110-137): p (3, n) unit
+    directions, e (3,) Sun -> observer unit vector, em its distance in au;
+    p + w p x (e x p) with w = SRS / em / max(p . (p + e), dlim)."""
+    p = np.asarray(p, dtype=np.float64)
+    e = np.asarray(e, dtype=np.float64)
+    qdqpe = np.maximum(np.sum(p * (p + e[:, None]), axis=0), dlim)
+    w = ERFA_SRS / em / qdqpe
+    eq = np.cross(e[None, :], p.T)
+    peq = np.cross(p.T, eq).T
+    return p + w[None] * peq
+
+
+def erfa_ab(pnat, v, s: float, bm1: float):
+    """Aberration (cpu/coords.py:139-163): natural direction (3, n) to
+    proper direction for an observer at barycentric velocity v (units of c),
+    distance s (au) from the Sun, bm1 = sqrt(1 - |v|^2); renormalized."""
+    pnat = np.asarray(pnat, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    pdv = v @ pnat
+    w1 = 1.0 + pdv / (1.0 + bm1)
+    w2 = ERFA_SRS / s
+    p = bm1 * pnat + np.outer(v, w1) + w2 * (v[:, None] - pdv[None] * pnat * bm1)
+    return p / np.linalg.norm(p, axis=0)
+
+
+def erfa_bcrs(eci, ctx: dict):
+    """CIRS proper directions of ICRS unit vectors (cpu/coords.py:176-215):
+    light deflection (limiter 1e-6), aberration, bias-precession-nutation."""
+    p = erfa_ld(eci, ctx["eh"], ctx["em"], 1e-6)
+    p = erfa_ab(p, ctx["v"], ctx["em"], ctx["bm1"])
+    return np.asarray(ctx["bpn"], dtype=np.float64) @ p
+
+
+def erfa_atioq_matrix(ctx: dict):
+    """CIRS -> (east, north, up) (cpu/coords.py:78-108): rotation by the
+    local Earth rotation angle, polar motion, then HA/Dec -> ENU."""
+    ce, se = np.cos(ctx["eral"]), np.sin(ctx["eral"])
+    c2h = np.array([[ce, se, 0.0], [-se, ce, 0.0], [0.0, 0.0, 1.0]])
+    sx, cx = np.sin(ctx["xpl"]), np.cos(ctx["xpl"])
+    sy, cy = np.sin(ctx["ypl"]), np.cos(ctx["ypl"])
+    pm = np.array([[cx, 0.0, sx], [sx * sy, cy, -cx * sy],
+                   [-sx * cy, sy, cx * cy]])
+    sphi, cphi = ctx["sphi"], ctx["cphi"]
+    enu = np.array([[0.0, 1.0, 0.0], [-sphi, 0.0, cphi], [cphi, 0.0, sphi]])
+    return enu @ pm @ c2h
 
 
 def normalized_error(v_test, v_ref, nant: int, nfeed: int):
