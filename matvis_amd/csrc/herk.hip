@@ -51,28 +51,62 @@ struct SliceCfg {
   int ovh;         // per-item cost beyond its K tiles, in K-tile units
 };
 
-// Active slice count for `ktiles` K tiles over `ntiles` tiles.  Pure integer
-// function of its arguments: the compute and reduce kernels agree on it.
+// Active slice count for `ktiles` K tiles over `ntiles` tiles: enough
+// slices that no chain exceeds cmax K tiles, then among the next 63 counts
+// (at most the capacity and ktiles) the one with the least cost = rounds of
+// resident slots x (K tiles per slice + per-item overhead); the smallest
+// count on ties.  Pure integer function of its arguments: the compute and
+// reduce kernels and the host agree on it.
+__host__ __device__ inline void slice_range(int ktiles, const SliceCfg &c,
+                                            int &lo, int &hi) {
+  lo = icdiv(ktiles, c.cmax);
+  lo = lo < 1 ? 1 : (lo > c.max_slices ? c.max_slices : lo);
+  hi = lo + 63;
+  if (hi > c.max_slices) hi = c.max_slices;
+  if (hi > ktiles) hi = ktiles > lo ? ktiles : lo;
+}
+__host__ __device__ inline int64_t slice_cost(int ktiles, int ntiles, int ns,
+                                              const SliceCfg &c) {
+  const int per_x = c.slots / kXcd > 0 ? c.slots / kXcd : 1;
+  const int rounds = icdiv(icdiv(ntiles * ns, kXcd), per_x);
+  return (int64_t)rounds * (icdiv(ktiles, ns) + c.ovh);
+}
 __host__ __device__ inline int active_slices(int ktiles, int ntiles,
                                              const SliceCfg &c) {
   if (ktiles <= 0) return 1;
-  int lo = icdiv(ktiles, c.cmax);
-  lo = lo < 1 ? 1 : (lo > c.max_slices ? c.max_slices : lo);
-  int hi = lo + 64;
-  if (hi > c.max_slices) hi = c.max_slices;
-  if (hi > ktiles) hi = ktiles > lo ? ktiles : lo;
-  const int per_x = c.slots / kXcd > 0 ? c.slots / kXcd : 1;
+  int lo, hi;
+  slice_range(ktiles, c, lo, hi);
   int best = lo;
   int64_t best_cost = INT64_MAX;
   for (int ns = lo; ns <= hi; ++ns) {
-    const int rounds = icdiv(icdiv(ntiles * ns, kXcd), per_x);
-    const int64_t cost = (int64_t)rounds * (icdiv(ktiles, ns) + c.ovh);
+    const int64_t cost = slice_cost(ktiles, ntiles, ns, c);
     if (cost < best_cost) {
       best_cost = cost;
       best = ns;
     }
   }
   return best;
+}
+// The same value with one candidate per lane of a full wave (wave-uniform
+// arguments): the least (cost, count) over the lanes.
+__device__ inline int active_slices_wave(int ktiles, int ntiles,
+                                         const SliceCfg &c) {
+  if (ktiles <= 0) return 1;
+  int lo, hi;
+  slice_range(ktiles, c, lo, hi);
+  const int lane = (int)(threadIdx.x & 63);
+  const int ns = lo + lane;
+  // cost < 2^48 (rounds and K tiles per slice are below 2^24); the lane in
+  // the low bits breaks ties toward the smaller count
+  uint64_t key = ns <= hi
+                     ? ((uint64_t)slice_cost(ktiles, ntiles, ns, c) << 6) | lane
+                     : UINT64_MAX;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t o = __shfl_xor(key, off, 64);
+    key = o < key ? o : key;
+  }
+  return lo + (int)(key & 63);
 }
 
 // Columns of each segment: the device count never exceeds the host bound the
@@ -149,7 +183,7 @@ __global__ __launch_bounds__(256) void herk_plain_kernel(
   const int64_t seg_len = seg_columns(seg_len_host, seg_len_dev);
   const int64_t kt_per_seg = cdiv(seg_len, (int64_t)kPK);
   const int ktiles = (int)(kt_per_seg * nseg);
-  const int ns = active_slices(ktiles, ntiles, cfg);
+  const int ns = active_slices_wave(ktiles, ntiles, cfg);
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
@@ -635,7 +669,7 @@ __global__ __launch_bounds__(SplitShape<TILE>::kThreads,
   const int64_t seg_len = seg_columns(P.seg_len_host, P.seg_len_dev);
   const int kps = (int)cdiv(seg_len, (int64_t)kSK);  // K tiles per segment
   const int ktiles = kps * P.nseg;
-  const int ns = active_slices(ktiles, P.ntiles, cfg);
+  const int ns = active_slices_wave(ktiles, P.ntiles, cfg);
   for_items((int64_t)P.ntiles * ns, [&](int64_t it) {
     split_item<NT, MAXORD, F16, TILE>(P, (int)(it / P.ntiles),
                                       (int)(it % P.ntiles), ns, ktiles, kps,
@@ -742,56 +776,74 @@ __global__ __launch_bounds__(SplitShape<TILE>::kThreads,
 
 // ------------------------------------------------------------ epilogues
 
-// Element e of upper tile `tile`: the sum of its `ns` slice partials in
+// Elements of upper tile `tile`: the sum of their `ns` slice partials in
 // FP64 (fixed order), scaled, stored or added into the accumulator.  With
 // `dpart` (split forms) the diagonal elements of diagonal tiles take the row
 // norms summed there instead of the MFMA sums.
 template <typename TP>
-__device__ __forceinline__ void reduce_element(
+__device__ __forceinline__ void reduce_pair(
     const cplx<TP> *__restrict__ part, const double *__restrict__ dpart,
     int ntiles, int nb, int ts, int tile, int64_t e, int ns, double scale,
     cplx<double> *__restrict__ vacc, int64_t ldv, int accumulate) {
+  // elements e and e + 1 (e even) of one tile: the same row, adjacent
+  // columns; eight independent sums over slices s = 8q + j per element,
+  // combined in a fixed order, so eight 16-byte loads are in flight
   const int64_t tsz = (int64_t)ts * ts;
   int bi, bj;
   tile_coords(tile, nb, bi, bj);
-  double sr = 0.0, si = 0.0;
-  if (dpart && bi == bj && e / ts == e % ts) {
-    for (int s = 0; s < ns; ++s)
-      sr += dpart[((int64_t)s * ntiles + tile) * ts + e / ts];
-  } else {
-    // four independent sums over slices s = 4q + j, combined in a fixed
-    // order: the loads of four slices are in flight at once
-    const cplx<TP> *src = part + (int64_t)tile * tsz + e;
-    const int64_t step = (int64_t)ntiles * tsz;
-    double r4[4] = {0.0, 0.0, 0.0, 0.0}, i4[4] = {0.0, 0.0, 0.0, 0.0};
-    int s = 0;
-    for (; s + 4 <= ns; s += 4) {
+  const cplx<TP> *src = part + (int64_t)tile * tsz + e;
+  const int64_t step = (int64_t)ntiles * tsz;
+  double re[2][8], im[2][8];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const cplx<TP> v = src[(int64_t)(s + j) * step];
-        r4[j] += (double)v.re;
-        i4[j] += (double)v.im;
-      }
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) re[k][j] = im[k][j] = 0.0;
+  int s = 0;
+  for (; s + 8 <= ns; s += 8) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const cplx<TP> *q = src + (int64_t)(s + j) * step;
+      const cplx<TP> a = q[0], b = q[1];
+      re[0][j] += (double)a.re;
+      im[0][j] += (double)a.im;
+      re[1][j] += (double)b.re;
+      im[1][j] += (double)b.im;
     }
-    for (; s < ns; ++s) {
-      const cplx<TP> v = src[(int64_t)s * step];
-      r4[0] += (double)v.re;
-      i4[0] += (double)v.im;
-    }
-    sr = (r4[0] + r4[1]) + (r4[2] + r4[3]);
-    si = (i4[0] + i4[1]) + (i4[2] + i4[3]);
   }
-  const int64_t row = (int64_t)bi * ts + e / ts, col = (int64_t)bj * ts + e % ts;
-  cplx<double> *dst = vacc + row * ldv + col;
-  if (accumulate) {
-    dst->re += sr * scale;
-    dst->im += si * scale;
-  } else {
-    *dst = cplx<double>{sr * scale, si * scale};
+  for (; s < ns; ++s) {
+    const cplx<TP> *q = src + (int64_t)s * step;
+    const cplx<TP> a = q[0], b = q[1];
+    re[0][0] += (double)a.re;
+    im[0][0] += (double)a.im;
+    re[1][0] += (double)b.re;
+    im[1][0] += (double)b.im;
+  }
+  const int64_t r = e / ts, c = e % ts;
+  cplx<double> *dst = vacc + ((int64_t)bi * ts + r) * ldv + (int64_t)bj * ts + c;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    double sr = ((re[k][0] + re[k][1]) + (re[k][2] + re[k][3])) +
+                ((re[k][4] + re[k][5]) + (re[k][6] + re[k][7]));
+    double si = ((im[k][0] + im[k][1]) + (im[k][2] + im[k][3])) +
+                ((im[k][4] + im[k][5]) + (im[k][6] + im[k][7]));
+    if (dpart && bi == bj && r == c + k) {
+      // the diagonal: the FP64 sum of |terms|^2 the compute kernel kept
+      sr = 0.0;
+      si = 0.0;
+      for (int t = 0; t < ns; ++t)
+        sr += dpart[((int64_t)t * ntiles + tile) * ts + r];
+    }
+    if (accumulate) {
+      dst[k].re += sr * scale;
+      dst[k].im += si * scale;
+    } else {
+      dst[k] = cplx<double>{sr * scale, si * scale};
+    }
   }
 }
 
-// grid (ts^2 / 256, ntiles): one thread per element of each upper tile
+// grid (ts^2 / 512, ntiles): one thread per two elements of each upper
+// tile; the slice count is found once per block
 template <typename TP>
 __global__ __launch_bounds__(256) void herk_reduce_kernel(
     const cplx<TP> *__restrict__ part, const double *__restrict__ dpart,
@@ -799,29 +851,35 @@ __global__ __launch_bounds__(256) void herk_reduce_kernel(
     const int32_t *__restrict__ seg_len_dev, int nseg, int kc, SliceCfg cfg,
     double scale, cplx<double> *__restrict__ vacc, int64_t ldv,
     int accumulate) {
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  __shared__ int ns_shared;
+  if (threadIdx.x < 64) {
+    const int64_t seg_len = seg_columns(seg_len_host, seg_len_dev);
+    const int ktiles = (int)(cdiv(seg_len, (int64_t)kc) * nseg);
+    const int ns = active_slices_wave(ktiles, ntiles, cfg);
+    if (threadIdx.x == 0) ns_shared = ns;
+  }
+  __syncthreads();
+  const int64_t e = 2 * ((int64_t)blockIdx.x * 256 + threadIdx.x);
   if (e >= (int64_t)ts * ts) return;
-  const int64_t seg_len = seg_columns(seg_len_host, seg_len_dev);
-  const int ktiles = (int)(cdiv(seg_len, (int64_t)kc) * nseg);
-  const int ns = active_slices(ktiles, ntiles, cfg);
-  reduce_element(part, dpart, ntiles, nb, ts, (int)blockIdx.y, e, ns, scale,
-                 vacc, ldv, accumulate);
+  reduce_pair(part, dpart, ntiles, nb, ts, (int)blockIdx.y, e, ns_shared,
+              scale, vacc, ldv, accumulate);
 }
 
-// grid (ts^2 / 256, total tiles of all problems)
+// Grouped reduce: blockIdx.y runs over the tiles of all problems (prefix
+// offsets from the plan); two elements per thread as above.
 __global__ __launch_bounds__(256) void herk_group_reduce_kernel(
     const SplitProblem *__restrict__ probs, int nprob, int ts,
     GroupPlan plan) {
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t e = 2 * ((int64_t)blockIdx.x * 256 + threadIdx.x);
   if (e >= (int64_t)ts * ts) return;
   // the grid covers the tiles of the problems' largest shapes (a graph
   // replayed with smaller ones keeps it); tiles past this plan's total idle
   if ((int32_t)blockIdx.y >= plan.tile_off[nprob]) return;
   const int p = find_segment(plan.tile_off, nprob, (int32_t)blockIdx.y);
   const SplitProblem &P = probs[p];
-  reduce_element(P.part, P.dpart, P.ntiles, P.nb, ts,
-                 (int)blockIdx.y - plan.tile_off[p], e, plan.ns[p], P.scale,
-                 P.vacc, P.ldv, P.accumulate);
+  reduce_pair(P.part, P.dpart, P.ntiles, P.nb, ts,
+              (int)blockIdx.y - plan.tile_off[p], e, plan.ns[p], P.scale,
+              P.vacc, P.ldv, P.accumulate);
 }
 
 __global__ __launch_bounds__(256) void vacc_sum_kernel(
@@ -1072,7 +1130,7 @@ int herk_run(int phases, int form, int nterms, int nprod, const void *z0,
     return MV_ERR_HIP;
   }
   hipStream_t st = as_stream(stream);
-  const dim3 rgrid((unsigned)cdiv((int64_t)ts * ts, (int64_t)256),
+  const dim3 rgrid((unsigned)cdiv((int64_t)ts * ts / 2, (int64_t)256),
                    (unsigned)ntiles);
   if (form == MV_FORM_C128) {
     if (phases & kPhaseCompute) {
@@ -1317,7 +1375,7 @@ extern "C" int mv_herk_grouped(int form, int nterms, int nprod, int tile,
 #undef MV_GROUP_TILE
 #undef MV_GROUP_LAUNCH
   if (int e = check_launch("mv_herk_grouped/split")) return e;
-  herk_group_reduce_kernel<<<dim3((unsigned)cdiv((int64_t)tile * tile,
+  herk_group_reduce_kernel<<<dim3((unsigned)cdiv((int64_t)tile * tile / 2,
                                                  (int64_t)256),
                                   (unsigned)total_tiles),
                              256, 0, st>>>(probs, nprob, tile, plan);

@@ -240,9 +240,9 @@ __global__ __launch_bounds__(kThreads) void bcrs_kernel(
   const double inv = 1.0 / sqrt(r2);
   for (int k = 0; k < 3; ++k) q[k] *= inv;
   for (int r = 0; r < 3; ++r)
-    out[r * ld_out + i] = __fma_rn(c.bpn.m[3 * r + 2], q[2],
-                                   __fma_rn(c.bpn.m[3 * r + 1], q[1],
-                                            c.bpn.m[3 * r] * q[0]));
+    out[r * ld_out + i] =
+        fma(c.bpn.m[3 * r + 2], q[2],
+            fma(c.bpn.m[3 * r + 1], q[1], c.bpn.m[3 * r] * q[0]));
 }
 
 }  // namespace
