@@ -37,6 +37,9 @@ NFREQ = 20
 NTIMES_POOL = 30
 MFMA_BF16_PEAK = 2516.6e12
 FP64_PEAK = 78.6e12
+#: dense MFMA peaks of MI355X (TFLOP/s) for each form's arithmetic
+PEAK_TFLOPS = {"FORM_SPLIT_BF16": 2516.6, "FORM_SPLIT_F16": 2516.6,
+               "FORM_C128": 78.6, "FORM_C64": 157.3}
 
 
 def build_workload(nsrc=NSRC, nant=NANT, nfreq=NFREQ, seed=1):
@@ -62,6 +65,7 @@ FORMS = {
     "bf16x3": ("FORM_SPLIT_BF16", 3, 6),
     "f16": ("FORM_SPLIT_F16", 2, 3),
     "fp64": ("FORM_C128", 1, 1),
+    "fp32": ("FORM_C64", 1, 1),
 }
 FORM_LABEL = {
     "bf16": "split-BF16 (2 terms, 3 products)",
@@ -69,6 +73,7 @@ FORM_LABEL = {
     "bf16x3": "split-BF16 (3 terms, 6 products)",
     "f16": "split-FP16 (2 terms, 3 products)",
     "fp64": "plain FP64",
+    "fp32": "plain complex64 (precision=1, the reference's default)",
 }
 
 
@@ -409,7 +414,8 @@ def serving_requests(seconds: float, rate: float, seed: int = 11):
 
 
 def serving_bench(seconds: float = 20.0, rate: float = 20.0,
-                  drain: float = 20.0, seed: int = 11, max_active: int = 16):
+                  drain: float = 20.0, seed: int = 11, max_active: int = 16,
+                  policy: str = "fifo"):
     """The resident server under a Poisson stream (arrivals for `seconds`,
     then up to `drain` s to finish).  Reports latency of the finished
     requests, lower bounds that count unfinished ones at their age, the
@@ -433,7 +439,7 @@ def serving_bench(seconds: float = 20.0, rate: float = 20.0,
     for lb in {s[2] for s in stream}:
         idx = [i for i, s in enumerate(stream) if s[2] == lb][:10]
         keep.update(sorted(idx, key=lambda i: stream[i][3])[:2])
-    srv = VisibilityServer(max_active=max_active)
+    srv = VisibilityServer(max_active=max_active, policy=policy)
     reqs = [SimRequest(kw, keep=i in keep, label=lb)
             for i, (_, kw, lb, _) in enumerate(stream)]
     srv.start()
@@ -486,6 +492,7 @@ def serving_bench(seconds: float = 20.0, rate: float = 20.0,
         "finished_per_s": st["requests"] / elapsed,
         "by_class": st.get("by_label", {}),
         "max_active": max_active,
+        "policy": policy,
     }
     del srv
     torch.cuda.empty_cache()
@@ -601,7 +608,7 @@ def main():
     ap.add_argument("--nfreq", type=int, default=NFREQ)
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline (for profiling runs)")
-    ap.add_argument("--forms", default="bf16,bf16x4,f16,fp64",
+    ap.add_argument("--forms", default="bf16,bf16x4,f16,fp32,fp64",
                     help="comma list of " + ",".join(FORMS))
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the grouped-HERK batch (256 mixed instances)")
@@ -651,7 +658,8 @@ def main():
         fname, nt, npd = FORMS[name]
         form = getattr(L, fname)
         log(f"form {name}: setting up")
-        eng = make_engine(w, form, nt, npd)
+        eng = make_engine(w, form, nt, npd,
+                          precision=1 if name == "fp32" else 2)
         eng.coords_eci_host = eci_h
         log(f"form {name}: timing {args.steps} steps after {args.warmup} "
             "warmup")
@@ -675,13 +683,16 @@ def main():
             herk_exec_flops=npd * alg_flops,
             herk_tflops=alg_flops / herk_ms / 1e9,
             herk_exec_tflops=npd * alg_flops / herk_ms / 1e9,
+            herk_peak_tflops=PEAK_TFLOPS[FORMS[name][0]],
+            herk_frac_of_peak=npd * alg_flops / herk_ms / 1e9
+            / PEAK_TFLOPS[FORMS[name][0]],
             n_up_mean=float(np.mean(nups)),
         )
         # keep one instance (time 0, first frequencies) for the error figures
         out = eng.new_output()
         eng.compute_time(0, out)
         torch.cuda.synchronize()
-        outs[name] = out[:2].cpu().numpy()
+        outs[name] = out[:2].cpu().numpy().astype(np.complex128)
         del eng
         torch.cuda.empty_cache()
 
@@ -764,7 +775,10 @@ def main():
         line["diffuse"] = diffuse_bench()
     if not args.no_serving:
         line["serving"] = serving_bench(args.serve_seconds, args.serve_rate,
-                                        args.serve_drain)
+                                        args.serve_drain, policy="fifo")
+        line["serving_shortest_first"] = serving_bench(
+            args.serve_seconds, args.serve_rate, args.serve_drain,
+            policy="shortest")
     if "fp64" in results:
         line["speedup_over_fp64"] = {
             n: results[n]["vis_per_s"] / results["fp64"]["vis_per_s"]

@@ -66,9 +66,26 @@ class SimRequest:
     t_admit: float = 0.0
     t_done: float = 0.0
     visibilities: int = 0
+    #: projected device work (set when the shortest-first policy needs it)
+    work: float | None = None
     result: np.ndarray | None = None
     error: Exception | None = None
     done: threading.Event = field(default_factory=threading.Event)
+
+
+def projected_work(engine_kwargs: dict) -> float:
+    """Relative device work of a request: instances x m^2 x sources (the
+    HERK term that dominates), FP64 forms weighted 5x."""
+    nant = len(engine_kwargs["antpos"])
+    nf = 2 if engine_kwargs.get("polarized") else 1
+    nsrc = np.asarray(engine_kwargs["fluxes"]).shape[0]
+    nfreq = np.atleast_1d(engine_kwargs["freqs"]).size
+    times = engine_kwargs["times"]
+    ntimes = len(np.atleast_1d(getattr(times, "jd", times)))
+    split = "Split" in engine_kwargs.get("matprod_method", "GPUMatMul")
+    fp64 = not split and engine_kwargs.get("precision", 1) == 2
+    return float(nfreq * ntimes * (nf * nant) ** 2 * nf * nsrc
+                 * (5.0 if fp64 else 1.0))
 
 
 def projected_bytes(engine_kwargs: dict) -> int:
@@ -170,14 +187,21 @@ class VisibilityServer:
     """See the module docstring.  ``hbm_budget``: bytes of device memory the
     resident requests may hold (default: 80% of free memory); ``max_active``:
     at most this many requests resident at once; ``graphs``: capture the
-    grouped HERK launches."""
+    grouped HERK launches; ``policy``: admission order, ``"fifo"`` (arrival)
+    or ``"shortest"`` (least projected work first, aged: a request's work
+    counts as work / (1 + wait / ``age_s``), so long waits are admitted)."""
 
     def __init__(self, hbm_budget: int | None = None, max_active: int = 8,
-                 graphs: bool = True):
+                 graphs: bool = True, policy: str = "fifo",
+                 age_s: float = 10.0):
         D.require_gpu()
+        if policy not in ("fifo", "shortest"):
+            raise ValueError("policy must be 'fifo' or 'shortest'")
         self.budget = int(hbm_budget or 0.8 * D.free_memory())
         self.max_active = max_active
         self.graphs = graphs
+        self.policy = policy
+        self.age_s = float(age_s)
         self.queue: list[SimRequest] = []
         self.active: list[_Active] = []
         self.finished: list[SimRequest] = []
@@ -296,10 +320,11 @@ class VisibilityServer:
             with self._lock:
                 if not self.queue:
                     return
-                need = projected_bytes(self.queue[0].kwargs)
+                i = self._next_index()
+                need = projected_bytes(self.queue[i].kwargs)
                 if self.active and self.used + need > self.budget:
                     return
-                req = self.queue.pop(0)
+                req = self.queue.pop(i)
             try:
                 engine, ntimes = build_engine(**req.kwargs,
                                               beam_cache=self._beams)
@@ -313,6 +338,17 @@ class VisibilityServer:
             req.t_admit = time.perf_counter()
             self.used += need
             self.active.append(_Active(req, engine, ntimes, need))
+
+    def _next_index(self) -> int:
+        """Queue position of the request to admit next (lock held)."""
+        if self.policy == "fifo" or len(self.queue) == 1:
+            return 0
+        now = time.perf_counter()
+        for r in self.queue:
+            if r.work is None:
+                r.work = projected_work(r.kwargs)
+        return min(range(len(self.queue)), key=lambda i: self.queue[i].work
+                   / (1.0 + (now - self.queue[i].t_submit) / self.age_s))
 
     def _grouped(self, items):
         from .gpu.herk_group import GroupedBatch

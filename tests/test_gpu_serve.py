@@ -147,3 +147,22 @@ def test_requests_admitted_while_others_run():
         assert np.array_equal(r.result, a), s
     # the later requests ran while the long one was resident
     assert all(r.t_admit < first.t_done for r in rest)
+
+
+def test_shortest_first_policy():
+    """policy='shortest' with one resident slot admits the request with the
+    least projected work first; results still equal the alone results."""
+    print("serve shortest-first", flush=True)
+    specs = [MIX[2], MIX[4], MIX[1]]
+    from matvis_amd.serve import projected_work
+    works = [projected_work(_kwargs(*s)) for s in specs]
+    assert len(set(works)) == 3
+    alone = [_serve([s])[1][0] for s in specs]
+    srv = VisibilityServer(max_active=1, policy="shortest")
+    reqs = [srv.submit(SimRequest(_kwargs(*s))) for s in specs]
+    srv.run_until_idle(timeout=300)
+    order = sorted(range(3), key=lambda i: reqs[i].t_admit)
+    assert order == sorted(range(3), key=lambda i: works[i]), (order, works)
+    assert order != [0, 1, 2]  # not the arrival order
+    for r, a in zip(reqs, alone):
+        assert r.error is None and np.array_equal(r.result, a)
