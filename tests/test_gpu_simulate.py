@@ -362,3 +362,73 @@ def test_diffuse_full_size_chunk_sums():
     assert e_chunk < 1e-12
     assert e_bf16 < 1e-6
     assert e_fp32 > e_chunk
+
+
+# ------------------------------------------------------------ full size
+# The headline size (350 hex antennas x 100,000 sources, polarized, bicubic
+# complex64 E-field beam), where the oracle is too slow: properties that hold
+# at any size, and the split form against the FP64 form.
+
+
+@pytest.fixture(scope="module")
+def full_size():
+    from matvis_amd import synth
+
+    nant, nsrc = 350, 100_000
+    antpos = synth.hex_array(nant, 14.6, 0.02, seed=1)
+    ra, dec = synth.random_sky(nsrc, 2, -90.0, 30.0)
+    flux = synth.lognormal_fluxes(nsrc, 3, 0.0, 1.0)
+    freqs = np.array([100e6, 200e6])
+    times = synth.times_spaced(1, 2.0)
+    loc = synth.hera_location()
+    rot = astro.rotation_matrices(times, loc)
+    beam = synth.harmonic_efield_beam(seed=4, naz=360, nza=91)
+    kw = dict(antpos=antpos, freqs=freqs, times=times,
+              skycoords=astro.SkyCoord(ra, dec), telescope_loc=loc,
+              beam_list=[beam], polarized=True, precision=2,
+              coord_method="CoordinateRotationMatrix",
+              coord_method_params={"rotations": rot},
+              beam_spline_opts={"order": 3})
+    out = {}
+    for method in ("GPUMatMul", "GPUMatMulSplitBF16"):
+        print(f"full size {method}", flush=True)
+        for s in (1.0, 4.0):
+            f = np.repeat(flux[:, None] * s, 2, 1)
+            out[method, s] = simulate_multi(fluxes=f, matprod_method=method,
+                                            **kw)
+    return nant, out
+
+
+@pytest.mark.parametrize("method", ["GPUMatMul", "GPUMatMulSplitBF16"])
+def test_full_size_hermitian_and_flux_linear(full_size, method):
+    nant, out = full_size
+    v = out[method, 1.0]  # (nfreq, ntimes, nant^2, 2, 2)
+    w = v.reshape(2, 1, nant, nant, 2, 2)
+    # V[i, j, p, q] == conj(V[j, i, q, p]) exactly (both come from one
+    # upper-triangle element) except on the self-correlations (i = j,
+    # p = q); those are real (the split forms write the exact FP64 row norm;
+    # FP64 cancels re*im - im*re to rounding) and positive
+    herm = np.conj(w.transpose(0, 1, 3, 2, 5, 4))
+    idx = np.arange(nant)
+    off = np.ones(w.shape, bool)
+    for p in range(2):
+        off[:, :, idx, idx, p, p] = False
+    np.testing.assert_array_equal(w[off], herm[off])
+    autos = np.diagonal(w[:, :, idx, idx], axis1=-2, axis2=-1)
+    assert np.all(autos.real > 0)
+    assert np.all(np.abs(autos.imag) <= 1e-13 * autos.real)
+    if method != "GPUMatMul":
+        assert np.all(autos.imag == 0)
+    # sqrt(4 I / 2) = 2 sqrt(I / 2): Z and its 16-bit split scale by 2
+    # exactly, V by 4
+    np.testing.assert_array_equal(out[method, 4.0], 4.0 * v)
+
+
+def test_full_size_split_within_1e6_of_fp64(full_size):
+    nant, out = full_size
+    for fi in range(2):
+        e = O.normalized_error(out["GPUMatMulSplitBF16", 1.0][fi],
+                               out["GPUMatMul", 1.0][fi], nant, 2)
+        print(f"full size freq {fi}: split-BF16 vs FP64 normalised max "
+              f"{e[0]:.2e} median {e[1]:.2e}", flush=True)
+        assert e[0] < 1e-6
