@@ -47,10 +47,14 @@ typedef void *mv_stream_t;
 #define MV_FORM_SPLIT_F16 3  /* Z split into fp16 terms, FP16 MFMA + FP32 acc */
 
 /* Copies the last error message of the calling thread into buf (truncated,
- * NUL-terminated) and returns the last status code (0 if none). */
+ * NUL-terminated) and returns the last status code (0 if none).  The Python
+ * layer raises the reference's exception types with these messages (the
+ * reference raises directly: src/matvis/core/coords.py:117-121,
+ * src/matvis/core/beams.py:203-207). */
 int mv_last_error(char *buf, size_t len);
 
-/* Library ABI version (incremented on any signature change). */
+/* Library ABI version (incremented on any signature change).  No reference
+ * counterpart: lets a binding check it loaded a matching library. */
 int mv_abi_version(void);
 
 /* ---------------------------------------------------------------- (a) ---- */
@@ -65,7 +69,9 @@ int mv_rotate(const double *eci, int64_t ld_eci, const double *rot9,
               int64_t nsrc, void *topo, int topo_is_f64, int64_t ld_topo,
               mv_stream_t stream);
 
-/* Bytes of workspace mv_rotate_cull needs for a chunk of chunk_len sources. */
+/* Bytes of workspace mv_rotate_cull needs for a chunk of chunk_len sources
+ * (the scan's block sums).  No reference counterpart: xp.where allocates
+ * its own (src/matvis/core/coords.py:115). */
 size_t mv_rotate_cull_workspace_bytes(int64_t chunk_len);
 
 /* Fused rotation + horizon cull (z > 0) + stable stream compaction of sources
@@ -190,7 +196,9 @@ int mv_zmat_fused(const double *antpos, int nant, double freq,
 
 /* Split a complex matrix (c128 or c64, layout as mv_zmat_fused output) into
  * nterms 16-bit terms of `form` (SPLIT_BF16 / SPLIT_F16), after multiplying
- * by `scale`.  Rows [0, rows), columns [0, cols) of each of nseg segments. */
+ * by `scale`.  Rows [0, rows), columns [0, cols) of each of nseg segments.
+ * For Z built outside mv_zmat_fused, e.g. by the plugin chain's ZMatrixCalc
+ * (src/matvis/core/getz.py:80-100) ahead of a split MatProd. */
 int mv_split_z(const void *z, int z_is_f64, int64_t rows, int nseg,
                int64_t cols, int64_t ldz, int64_t seg_stride, int form,
                int nterms, double scale, void *t0, void *t1, void *t2,
@@ -198,11 +206,13 @@ int mv_split_z(const void *z, int z_is_f64, int64_t rows, int nseg,
 
 /* ---------------------------------------------------------------- (d) ---- */
 
-/* Tile side (rows of V per workgroup tile) used by a HERK form. */
+/* Tile side (rows of V per workgroup tile) used by a HERK form: the
+ * padding of the accumulator (ldv >= tile-rounded m).  No reference
+ * counterpart (cuBLAS writes the full m x m product). */
 int mv_herk_tile(int form);
 
 /* Workspace bytes for mv_herk with room for `nslices` K-slices at row
- * count m. */
+ * count m (partial tiles of the split-K).  No reference counterpart. */
 size_t mv_herk_workspace_bytes(int form, int m, int nslices);
 
 /* Slice capacity that covers every slice count mv_herk may pick for up to
@@ -214,7 +224,8 @@ int mv_herk_max_slices(int form, int m, int64_t kmax);
  * capacity of nslices (the kernels choose it on the device from the device
  * column count by the same rule): enough slices that no FP32 accumulation
  * chain exceeds the form's limit, then the one that best fills the GPU's
- * resident workgroup slots.  Returns the count, or a negative status. */
+ * resident workgroup slots.  Returns the count, or a negative status.  No
+ * reference counterpart (reporting and workspace planning). */
 int mv_herk_slices(int form, int nterms, int nprod, int m, int nseg,
                    int64_t seg_len, int nslices);
 
@@ -265,7 +276,9 @@ int mv_herk_reduce(int form, int nterms, int nprod, const void *z0,
                    double *vacc, int64_t ldv, int accumulate, void *workspace,
                    size_t ws_bytes, mv_stream_t stream);
 
-/* One instance of a grouped split HERK (mv_herk_grouped): plain integers
+/* One instance of a grouped split HERK (mv_herk_grouped; no reference
+ * counterpart, the reference runs one matprod per (time, frequency, chunk),
+ * src/matvis/gpu/gpu.py:240-250): plain integers
  * and device addresses so that a binding can pack an array of them
  * (96 bytes each; mv_herk_problem_bytes() reports the size).  Fields as in
  * mv_herk: z = term arrays (z[2] = 0 unless 3 terms), ldz / seg_stride /
@@ -284,7 +297,8 @@ typedef struct mv_herk_problem {
 
 size_t mv_herk_problem_bytes(void);
 
-/* Workspace bytes for mv_herk_grouped over these problems (host copy). */
+/* Workspace bytes for mv_herk_grouped over these problems (host copy).
+ * No reference counterpart. */
 size_t mv_herk_grouped_workspace_bytes(int form, int tile,
                                        const mv_herk_problem *problems,
                                        int nprob);
