@@ -1,6 +1,6 @@
 """Benchmark: visibilities/s of the per-time computation on one MI355X.
 
-Workload (north-star headline): 350 antennas on a 14.6 m hexagonal grid with
+Workload (the headline configuration): 350 antennas on a 14.6 m hexagonal grid with
 +-2 cm jitter, 100,000 point sources (RA uniform, sin(dec) uniform on
 [-90, +30] deg, log-normal(0, 1) Jy fluxes), polarized (nfeed = nax = 2),
 20 frequencies over 100-200 MHz, times 2 min apart, one shared complex64
@@ -628,7 +628,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(instances shard by (time, frequency); see DESIGN.md)",
+              "(instances shard by (time, frequency) across GPUs)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -173,7 +173,7 @@ int mv_zmat_parts(const void *sqrt_flux, const void *beam,
 
 /* Fused phase + Z: Z[a*nfeed+f, ax, s] = A[bidx(a), f, ax, s] * flux[s] *
  * exp(i 2 pi freq (b_a . crd_s) / c), written once, coalesced along s.
- * The north-star fusion of TauCalculator and ZMatrixCalc
+ * The fusion of TauCalculator and ZMatrixCalc
  * (src/matvis/core/tau.py:65-66, core/getz.py:80-100).
  * antpos: device (nant, 3) f64 metres.  crd: (3, ld_crd) f64.
  * flux: f64 sqrt(I/2); read as flux[idx[s]] when idx (int32, count, the

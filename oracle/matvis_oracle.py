@@ -2,8 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY.  The product package (``matvis_amd``) never imports
 this module.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` use it: as the checker the HIP path is
-compared against, and as the CPU baseline timed beside the GPU figures.
+``cpu_baseline`` leg of ``bench.py`` use it: as the reference result the HIP
+path is compared against, and as the CPU baseline timed beside the GPU figures.
 
 It is a numpy (float64 / complex128) restatement of matvis's CPU path at
 ``precision=2``.  Every function cites the reference file:line it follows
