@@ -14,7 +14,7 @@ into the API layout, results left in device memory.  Forms timed: split-BF16
 MFMA with FP32 accumulation per K-slice, FP64 cross-slice sums and an exact
 FP64 diagonal), its four-product variant, split-FP16 and plain FP64.
 
-Prints one JSON line.  `--gpus N` with N > 1 exits: this build targets one GPU.
+Prints one JSON line.  `--gpus N` with N > 1 exits: the library runs on one GPU.
 """
 
 from __future__ import annotations
@@ -627,7 +627,7 @@ def main():
                          "known-size kernel that calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: matvis_amd has no multi-GPU path "
               "(instances shard by (time, frequency) across GPUs)",
               file=sys.stderr)
         sys.exit(2)
