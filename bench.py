@@ -168,7 +168,8 @@ def batched_bench(ninst: int = 256, seed: int = 7, reps: int = 5):
     horizon, synthetic complex64 Z split to bf16 (2 terms, 3 products);
     tiles from the autotuner; the grouped launches captured in a HIP graph
     and replayed.  Compared with one plain complex64 HERK launch per
-    instance and one split launch per instance."""
+    instance, one split launch per instance, and the plain complex64 HERKs
+    of the same instances in one grouped, captured launch."""
     import torch
 
     import matvis_amd._device as D
@@ -249,10 +250,17 @@ def batched_bench(ninst: int = 256, seed: int = 7, reps: int = 5):
 
     t_plain = sequential(plain_all)
     t_split = sequential(split_all)
+    # the plain complex64 HERKs of the same instances as one grouped launch
+    from matvis_amd.gpu.herk_group import plain_tile
+    probs64 = [dict(p, terms=[z]) for p, z in zip(probs, zs)]
+    batch64 = GroupedBatch(L.FORM_C64, 1, 1, probs64,
+                           [plain_tile()] * ninst).capture()
+    t_group64 = sequential(batch64.replay)
+    del batch64, probs64
     # errors of the grouped result against the FP64 HERK on a sample
     batch.replay()
     torch.cuda.synchronize()
-    errs = []
+    errs, meds = [], []
     for i in rng.choice(ninst, 8, replace=False):
         p, z = probs[i], zs[i]
         h = Herk(L.FORM_C128, p["m"])
@@ -264,7 +272,10 @@ def batched_bench(ninst: int = 256, seed: int = 7, reps: int = 5):
         ref = torch.triu(v[:m, :m])
         got = torch.triu(p["vacc"][:m, :m])
         d = torch.sqrt(torch.diagonal(ref).real.abs())
-        errs.append(float((torch.abs(got - ref) / (d[:, None] * d[None])).max()))
+        rel = torch.triu(torch.abs(got - ref) / (d[:, None] * d[None]))
+        errs.append(float(rel.max()))
+        iu = torch.triu_indices(m, m, device=rel.device)
+        meds.append(float(torch.median(rel[iu[0], iu[1]])))
     del zs, probs, batch, herks
     torch.cuda.empty_cache()
     return {
@@ -275,9 +286,12 @@ def batched_bench(ninst: int = 256, seed: int = 7, reps: int = 5):
         "sequential_split_bf16_ms": t_split,
         "speedup_vs_sequential_plain": t_plain / t_group,
         "speedup_vs_sequential_split": t_split / t_group,
+        "grouped_graph_plain_c64_ms": t_group64,
+        "plain_c64_speedup_grouped_vs_sequential": t_plain / t_group64,
         "tiles": {str(k): tiles.count(k) for k in sorted(set(tiles))},
         "autotune_ms": tune,
         "max_err_vs_fp64_8_instances": max(errs),
+        "median_err_vs_fp64_8_instances": float(np.median(meds)),
     }
 
 

@@ -303,11 +303,14 @@ size_t mv_herk_grouped_workspace_bytes(int form, int tile,
                                        const mv_herk_problem *problems,
                                        int nprob);
 
-/* Many split-form HERKs of different sizes in one persistent launch (plus a
- * plan and a reduce launch): every problem's (tile, K-slice) items, problem-
- * major, are swept by one workgroup per resident slot, so small problems
- * share the GPU instead of each launch idling most of it.  `tile` = 128 or
- * 64 (rows per workgroup tile; 64 suits m <= 128).  problems_host is read
+/* Many HERKs of one form and different sizes in one persistent launch (plus
+ * a plan and a reduce launch): every problem's (tile, K-slice) items,
+ * problem-major, are swept by one workgroup per resident slot, so small
+ * problems share the GPU instead of each launch idling most of it.  Split
+ * forms: `tile` = 128 or 64 (rows per workgroup tile; 64 suits m <= 128);
+ * plain forms (MV_FORM_C128, MV_FORM_C64, nterms = 1, z[0] the complex Z):
+ * `tile` = 64.  A problem's slice count depends on that problem alone, so
+ * its result is the same in any group.  problems_host is read
  * for validation and launch geometry, problems_dev (the same array in device
  * memory) by the kernels, so a captured graph replays with whatever the
  * device array holds.  No reference counterpart: the reference runs one

@@ -170,128 +170,142 @@ constexpr int kPRow = kPK + 1;  // LDS row length (one element of padding)
 // 256 threads = 4 waves in a 2x2 arrangement, each wave a 32x32 sub-tile made
 // of 2x2 blocks of 16x16.  One 16x16x4 MFMA consumes two complex columns:
 // lane l supplies k-slot g = l / 16 -> column g / 2, part g % 2 (re / im).
+template <typename T> struct PlainLds {
+  cplx<T> a[2][kPT * kPRow];
+  cplx<T> b[2][kPT * kPRow];
+};
+
+// One (slice, tile) item of a plain HERK: the slice's K tiles staged through
+// two LDS stages, the partial tile written to `part`.
+template <typename T>
+__device__ __forceinline__ void plain_item(
+    const cplx<T> *__restrict__ z, int m, int64_t ldz, int64_t seg_stride,
+    int64_t seg_len, int64_t kt_per_seg, int nb, int ntiles, int ktiles,
+    int ns, int slice, int tile, cplx<T> *__restrict__ part, PlainLds<T> &L) {
+  typedef typename PlainMfma<T>::acc acc_t;
+  cplx<T>(*As)[kPT * kPRow] = L.a;
+  cplx<T>(*Bs)[kPT * kPRow] = L.b;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int lrow = tid >> 2, lcol = (tid & 3) * 4;
+  const int g = lane >> 4, rl = lane & 15;
+  const bool im_part = g & 1;
+  int bi, bj;
+  tile_coords(tile, nb, bi, bj);
+  const bool diag = bi == bj;
+  // sub-tiles below the diagonal or wholly in the row padding do no MFMA
+  const bool active = !(diag && wr > wc) && bi * kPT + wr * 32 < m &&
+                      bj * kPT + wc * 32 < m;
+  const int64_t kt0 = (int64_t)slice * ktiles / ns;
+  const int64_t kt1 = (int64_t)(slice + 1) * ktiles / ns;
+  const int64_t grow_a = (int64_t)bi * kPT + lrow;
+  const int64_t grow_b = (int64_t)bj * kPT + lrow;
+
+  acc_t acc_re[2][2], acc_im[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      acc_re[i][j] = acc_t{};
+      acc_im[i][j] = acc_t{};
+    }
+
+  cplx<T> ra[4], rb[4];
+  auto gload = [&](int64_t kt, int64_t grow, cplx<T> *r) {
+    const int64_t sg = kt / kt_per_seg;
+    const int64_t c0 = (kt % kt_per_seg) * kPK + lcol;
+    const cplx<T> *src = z + grow * ldz + sg * seg_stride + c0;
+    const bool row_ok = grow < m;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      r[q] = (row_ok && c0 + q < seg_len) ? src[q] : cplx<T>{(T)0, (T)0};
+  };
+  auto lstore = [&](cplx<T> *dst, const cplx<T> *r) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[lrow * kPRow + lcol + q] = r[q];
+  };
+
+  if (kt0 < kt1) {
+    gload(kt0, grow_a, ra);
+    lstore(As[0], ra);
+    if (!diag) {
+      gload(kt0, grow_b, rb);
+      lstore(Bs[0], rb);
+    }
+  }
+  lds_barrier();
+
+  for (int64_t kt = kt0; kt < kt1; ++kt) {
+    const int buf = (int)((kt - kt0) & 1);
+    const bool has_next = kt + 1 < kt1;
+    if (has_next) {
+      gload(kt + 1, grow_a, ra);
+      if (!diag) gload(kt + 1, grow_b, rb);
+    }
+    const cplx<T> *Ab = As[buf];
+    const cplx<T> *Bb = diag ? As[buf] : Bs[buf];
+    if (active) {
+#pragma unroll
+      for (int ks = 0; ks < kPK / 2; ++ks) {
+        const int col = 2 * ks + (g >> 1);
+        T a[2], bre[2], bim[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          cplx<T> v = Ab[(wr * 32 + i * 16 + rl) * kPRow + col];
+          a[i] = im_part ? v.im : v.re;
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          cplx<T> v = Bb[(wc * 32 + j * 16 + rl) * kPRow + col];
+          bre[j] = im_part ? v.im : v.re;
+          bim[j] = im_part ? -v.re : v.im;
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            acc_re[i][j] = PlainMfma<T>::op(a[i], bre[j], acc_re[i][j]);
+            acc_im[i][j] = PlainMfma<T>::op(a[i], bim[j], acc_im[i][j]);
+          }
+      }
+    }
+    if (has_next) {
+      lstore(As[buf ^ 1], ra);
+      if (!diag) lstore(Bs[buf ^ 1], rb);
+    }
+    lds_barrier();
+  }
+
+  // 16x16 accumulator: lane l = 16 g + rl, register r -> row
+  // PlainMfma<T>::row(g, r), column rl
+  cplx<T> *out = part + ((int64_t)slice * ntiles + tile) * kPT * kPT;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = wr * 32 + i * 16 + PlainMfma<T>::row(g, r);
+        const int col = wc * 32 + j * 16 + rl;
+        out[row * kPT + col] = cplx<T>{acc_re[i][j][r], acc_im[i][j][r]};
+      }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void herk_plain_kernel(
     const cplx<T> *__restrict__ z, int m, int64_t ldz, int nseg,
     int64_t seg_stride, int64_t seg_len_host,
     const int32_t *__restrict__ seg_len_dev, int nb, int ntiles,
     SliceCfg cfg, cplx<T> *__restrict__ part) {
-  typedef typename PlainMfma<T>::acc acc_t;
-  __shared__ cplx<T> As[2][kPT * kPRow];
-  __shared__ cplx<T> Bs[2][kPT * kPRow];
-
+  __shared__ PlainLds<T> L;
   const int64_t seg_len = seg_columns(seg_len_host, seg_len_dev);
   const int64_t kt_per_seg = cdiv(seg_len, (int64_t)kPK);
   const int ktiles = (int)(kt_per_seg * nseg);
   const int ns = active_slices_wave(ktiles, ntiles, cfg);
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
-  const int lrow = tid >> 2, lcol = (tid & 3) * 4;
-  const int g = lane >> 4, rl = lane & 15;
-  const bool im_part = g & 1;
-
   for_items((int64_t)ntiles * ns, [&](int64_t it) {
-    const int slice = (int)(it / ntiles), tile = (int)(it % ntiles);
-    int bi, bj;
-    tile_coords(tile, nb, bi, bj);
-    const bool diag = bi == bj;
-    // sub-tiles below the diagonal or wholly in the row padding do no MFMA
-    const bool active = !(diag && wr > wc) && bi * kPT + wr * 32 < m &&
-                        bj * kPT + wc * 32 < m;
-    const int64_t kt0 = (int64_t)slice * ktiles / ns;
-    const int64_t kt1 = (int64_t)(slice + 1) * ktiles / ns;
-    const int64_t grow_a = (int64_t)bi * kPT + lrow;
-    const int64_t grow_b = (int64_t)bj * kPT + lrow;
-
-    acc_t acc_re[2][2], acc_im[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        acc_re[i][j] = acc_t{};
-        acc_im[i][j] = acc_t{};
-      }
-
-    cplx<T> ra[4], rb[4];
-    auto gload = [&](int64_t kt, int64_t grow, cplx<T> *r) {
-      const int64_t sg = kt / kt_per_seg;
-      const int64_t c0 = (kt % kt_per_seg) * kPK + lcol;
-      const cplx<T> *src = z + grow * ldz + sg * seg_stride + c0;
-      const bool row_ok = grow < m;
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        r[q] = (row_ok && c0 + q < seg_len) ? src[q] : cplx<T>{(T)0, (T)0};
-    };
-    auto lstore = [&](cplx<T> *dst, const cplx<T> *r) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) dst[lrow * kPRow + lcol + q] = r[q];
-    };
-
-    if (kt0 < kt1) {
-      gload(kt0, grow_a, ra);
-      lstore(As[0], ra);
-      if (!diag) {
-        gload(kt0, grow_b, rb);
-        lstore(Bs[0], rb);
-      }
-    }
-    lds_barrier();
-
-    for (int64_t kt = kt0; kt < kt1; ++kt) {
-      const int buf = (int)((kt - kt0) & 1);
-      const bool has_next = kt + 1 < kt1;
-      if (has_next) {
-        gload(kt + 1, grow_a, ra);
-        if (!diag) gload(kt + 1, grow_b, rb);
-      }
-      const cplx<T> *Ab = As[buf];
-      const cplx<T> *Bb = diag ? As[buf] : Bs[buf];
-      if (active) {
-#pragma unroll
-        for (int ks = 0; ks < kPK / 2; ++ks) {
-          const int col = 2 * ks + (g >> 1);
-          T a[2], bre[2], bim[2];
-#pragma unroll
-          for (int i = 0; i < 2; ++i) {
-            cplx<T> v = Ab[(wr * 32 + i * 16 + rl) * kPRow + col];
-            a[i] = im_part ? v.im : v.re;
-          }
-#pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            cplx<T> v = Bb[(wc * 32 + j * 16 + rl) * kPRow + col];
-            bre[j] = im_part ? v.im : v.re;
-            bim[j] = im_part ? -v.re : v.im;
-          }
-#pragma unroll
-          for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-              acc_re[i][j] = PlainMfma<T>::op(a[i], bre[j], acc_re[i][j]);
-              acc_im[i][j] = PlainMfma<T>::op(a[i], bim[j], acc_im[i][j]);
-            }
-        }
-      }
-      if (has_next) {
-        lstore(As[buf ^ 1], ra);
-        if (!diag) lstore(Bs[buf ^ 1], rb);
-      }
-      lds_barrier();
-    }
-
-    // 16x16 accumulator: lane l = 16 g + rl, register r -> row
-    // PlainMfma<T>::row(g, r), column rl
-    cplx<T> *out = part + ((int64_t)slice * ntiles + tile) * kPT * kPT;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = wr * 32 + i * 16 + PlainMfma<T>::row(g, r);
-          const int col = wc * 32 + j * 16 + rl;
-          out[row * kPT + col] = cplx<T>{acc_re[i][j][r], acc_im[i][j][r]};
-        }
+    plain_item<T>(z, m, ldz, seg_stride, seg_len, kt_per_seg, nb, ntiles,
+                  ktiles, ns, (int)(it / ntiles), (int)(it % ntiles), part,
+                  L);
   });
 }
 
@@ -688,13 +702,15 @@ struct GroupPlan {
 };
 
 // Single block: every problem's internal descriptor (tile grid, partial
-// areas carved from `parts` in problem order: cap x ntiles x (ts^2 + ts)
-// x 8 bytes each), its slice count (enough slices that no FP32 chain exceeds
-// cmax K tiles, at most its capacity) and the prefix offsets.
+// areas carved from `parts` in problem order: cap x ntiles x ts^2 partial
+// elements of part_bytes each, then, with row_norms, cap x ntiles x ts FP64
+// row norms), its slice count (enough slices that no chain exceeds cmax K
+// tiles, at most its capacity: a function of the problem alone) and the
+// prefix offsets.
 __global__ __launch_bounds__(256) void herk_group_plan_kernel(
     const mv_herk_problem *__restrict__ pub, int nprob, int ts, int kc,
-    int cmax, SplitProblem *__restrict__ probs, char *__restrict__ parts,
-    GroupPlan plan) {
+    int cmax, int part_bytes, int row_norms, SplitProblem *__restrict__ probs,
+    char *__restrict__ parts, GroupPlan plan) {
   for (int p = threadIdx.x; p < nprob; p += blockDim.x) {
     const mv_herk_problem &q = pub[p];
     SplitProblem P;
@@ -728,9 +744,9 @@ __global__ __launch_bounds__(256) void herk_group_plan_kernel(
       SplitProblem &P = probs[p];
       const int64_t cap = pub[p].nslices;
       P.part = (cplx<float> *)(parts + bytes);
-      bytes += cap * P.ntiles * (int64_t)ts * ts * 8;
-      P.dpart = (double *)(parts + bytes);
-      bytes += cap * P.ntiles * (int64_t)ts * 8;
+      bytes += cap * P.ntiles * (int64_t)ts * ts * part_bytes;
+      P.dpart = row_norms ? (double *)(parts + bytes) : nullptr;
+      if (row_norms) bytes += cap * P.ntiles * (int64_t)ts * 8;
       plan.item_off[p] = items;
       plan.tile_off[p] = tiles;
       items += (int64_t)P.ntiles * plan.ns[p];
@@ -771,6 +787,27 @@ __global__ __launch_bounds__(SplitShape<TILE>::kThreads,
     split_item<NT, MAXORD, F16, TILE>(P, (int)(local / P.ntiles),
                                       (int)(local % P.ntiles), ns,
                                       kps * P.nseg, kps, seg_len, L0, L1);
+  });
+}
+
+// Grouped plain HERK: the items of all problems, as the grouped split
+// kernel; each problem's Z is its term pointer 0.
+template <typename T>
+__global__ __launch_bounds__(256) void herk_plain_grouped_kernel(
+    const SplitProblem *__restrict__ probs, int nprob, GroupPlan plan) {
+  __shared__ PlainLds<T> L;
+  const int64_t items = plan.item_off[nprob];
+  for_items(items, [&](int64_t it) {
+    const int p = find_segment(plan.item_off, nprob, it);
+    const SplitProblem P = probs[p];
+    const int ns = plan.ns[p];
+    const int64_t local = it - plan.item_off[p];
+    const int64_t seg_len = seg_columns(P.seg_len_host, P.seg_len_dev);
+    const int64_t kt_per_seg = cdiv(seg_len, (int64_t)kPK);
+    plain_item<T>((const cplx<T> *)P.t[0], P.m, P.ldz, P.seg_stride, seg_len,
+                  kt_per_seg, P.nb, P.ntiles, (int)(kt_per_seg * P.nseg), ns,
+                  (int)(local / P.ntiles), (int)(local % P.ntiles),
+                  (cplx<T> *)P.part, L);
   });
 }
 
@@ -867,6 +904,7 @@ __global__ __launch_bounds__(256) void herk_reduce_kernel(
 
 // Grouped reduce: blockIdx.y runs over the tiles of all problems (prefix
 // offsets from the plan); two elements per thread as above.
+template <typename TP>
 __global__ __launch_bounds__(256) void herk_group_reduce_kernel(
     const SplitProblem *__restrict__ probs, int nprob, int ts,
     GroupPlan plan) {
@@ -877,7 +915,7 @@ __global__ __launch_bounds__(256) void herk_group_reduce_kernel(
   if ((int32_t)blockIdx.y >= plan.tile_off[nprob]) return;
   const int p = find_segment(plan.tile_off, nprob, (int32_t)blockIdx.y);
   const SplitProblem &P = probs[p];
-  reduce_pair(P.part, P.dpart, P.ntiles, P.nb, ts,
+  reduce_pair((const cplx<TP> *)P.part, P.dpart, P.ntiles, P.nb, ts,
               (int)blockIdx.y - plan.tile_off[p], e, plan.ns[p], P.scale,
               P.vacc, P.ldv, P.accumulate);
 }
@@ -1252,7 +1290,14 @@ struct GroupLayout {
   int64_t probs, ns, item_off, tile_off, parts, total;
 };
 
-GroupLayout group_layout(int ts, const mv_herk_problem *q, int nprob) {
+// Bytes per partial element and whether FP64 row norms are kept, per form.
+bool is_split_form(int form) {
+  return form == MV_FORM_SPLIT_BF16 || form == MV_FORM_SPLIT_F16;
+}
+int part_bytes(int form) { return form == MV_FORM_C128 ? 16 : 8; }
+
+GroupLayout group_layout(int form, int ts, const mv_herk_problem *q,
+                         int nprob) {
   GroupLayout g;
   g.probs = 0;
   g.ns = align256((int64_t)nprob * sizeof(SplitProblem));
@@ -1263,7 +1308,9 @@ GroupLayout group_layout(int ts, const mv_herk_problem *q, int nprob) {
   for (int i = 0; i < nprob; ++i) {
     const int64_t nb = cdiv((int64_t)q[i].m, (int64_t)ts);
     const int64_t ntiles = nb * (nb + 1) / 2;
-    bytes += (int64_t)q[i].nslices * ntiles * ((int64_t)ts * ts + ts) * 8;
+    bytes += (int64_t)q[i].nslices * ntiles *
+             ((int64_t)ts * ts * part_bytes(form) +
+              (is_split_form(form) ? (int64_t)ts * 8 : 0));
   }
   g.total = g.parts + bytes;
   return g;
@@ -1279,12 +1326,18 @@ extern "C" size_t mv_herk_problem_bytes(void) {
   return sizeof(mv_herk_problem);
 }
 
+namespace {
+bool group_tile_ok(int form, int tile) {
+  return is_split_form(form) ? (tile == 64 || tile == kST)
+                             : ((form == MV_FORM_C128 || form == MV_FORM_C64) &&
+                                tile == kPT);
+}
+}  // namespace
+
 extern "C" size_t mv_herk_grouped_workspace_bytes(
     int form, int tile, const mv_herk_problem *problems, int nprob) {
-  if ((form != MV_FORM_SPLIT_BF16 && form != MV_FORM_SPLIT_F16) ||
-      (tile != 64 && tile != kST) || !problems || nprob < 1)
-    return 0;
-  return (size_t)group_layout(tile, problems, nprob).total;
+  if (!group_tile_ok(form, tile) || !problems || nprob < 1) return 0;
+  return (size_t)group_layout(form, tile, problems, nprob).total;
 }
 
 extern "C" int mv_herk_grouped(int form, int nterms, int nprod, int tile,
@@ -1292,13 +1345,15 @@ extern "C" int mv_herk_grouped(int form, int nterms, int nprod, int tile,
                                const mv_herk_problem *problems_dev, int nprob,
                                void *workspace, size_t ws_bytes,
                                mv_stream_t stream) {
-  MV_REQUIRE(form == MV_FORM_SPLIT_BF16 || form == MV_FORM_SPLIT_F16,
-             "mv_herk_grouped: split forms only (form %d)", form);
-  MV_REQUIRE(tile == 64 || tile == kST, "mv_herk_grouped: tile %d (64 or %d)",
-             tile, kST);
-  const int maxord = split_maxord(nterms, nprod);
-  MV_REQUIRE(maxord >= 0,
-             "mv_herk_grouped: unsupported split (nterms=%d, nprod=%d)",
+  const bool split = is_split_form(form);
+  MV_REQUIRE(split || form == MV_FORM_C128 || form == MV_FORM_C64,
+             "mv_herk_grouped: unknown form %d", form);
+  MV_REQUIRE(group_tile_ok(form, tile),
+             "mv_herk_grouped: tile %d (split forms: 64 or %d; plain: %d)",
+             tile, kST, kPT);
+  const int maxord = split ? split_maxord(nterms, nprod) : 0;
+  MV_REQUIRE(split ? maxord >= 0 : nterms == 1,
+             "mv_herk_grouped: unsupported terms (nterms=%d, nprod=%d)",
              nterms, nprod);
   MV_REQUIRE(problems_host && problems_dev && workspace && nprob >= 1,
              "mv_herk_grouped: null pointer or no problems");
@@ -1309,17 +1364,26 @@ extern "C" int mv_herk_grouped(int form, int nterms, int nprod, int tile,
                "mv_herk_grouped: problem %d: bad sizes m=%d nseg=%d "
                "seg_len=%lld nslices=%d", i, q.m, q.nseg,
                (long long)q.seg_len, q.nslices);
-    MV_REQUIRE(q.z[0] && q.z[1] && (nterms < 3 || q.z[2]) && q.vacc,
+    MV_REQUIRE(q.z[0] && (!split || q.z[1]) && (nterms < 3 || q.z[2]) &&
+                   q.vacc,
                "mv_herk_grouped: problem %d: null pointer", i);
-    const int64_t padded = cdiv(q.seg_len, (int64_t)kSK) * kSK;
-    MV_REQUIRE(q.ldz % 4 == 0 && (q.nseg == 1 || q.seg_stride % 4 == 0) &&
-                   ((q.z[0] | q.z[1] | (q.z[2] ? q.z[2] : q.z[0])) % 16) == 0,
-               "mv_herk_grouped: problem %d: terms need 16-byte aligned rows",
-               i);
-    MV_REQUIRE((q.nseg == 1 || q.seg_stride >= padded) &&
-                   q.ldz >= (int64_t)(q.nseg - 1) * q.seg_stride + padded,
-               "mv_herk_grouped: problem %d: segments must hold "
-               "round_up(seg_len, %d) columns", i, kSK);
+    if (split) {
+      const int64_t padded = cdiv(q.seg_len, (int64_t)kSK) * kSK;
+      MV_REQUIRE(q.ldz % 4 == 0 && (q.nseg == 1 || q.seg_stride % 4 == 0) &&
+                     ((q.z[0] | q.z[1] | (q.z[2] ? q.z[2] : q.z[0])) % 16) ==
+                         0,
+                 "mv_herk_grouped: problem %d: terms need 16-byte aligned "
+                 "rows", i);
+      MV_REQUIRE((q.nseg == 1 || q.seg_stride >= padded) &&
+                     q.ldz >= (int64_t)(q.nseg - 1) * q.seg_stride + padded,
+                 "mv_herk_grouped: problem %d: segments must hold "
+                 "round_up(seg_len, %d) columns", i, kSK);
+    } else {
+      MV_REQUIRE((q.nseg == 1 || q.seg_stride >= q.seg_len) &&
+                     q.ldz >= (int64_t)(q.nseg - 1) * q.seg_stride + q.seg_len,
+                 "mv_herk_grouped: problem %d: segments overlap or ldz too "
+                 "small", i);
+    }
     const int64_t nb = cdiv((int64_t)q.m, (int64_t)tile);
     MV_REQUIRE(q.ldv >= nb * tile,
                "mv_herk_grouped: problem %d: ldv %lld < %lld", i,
@@ -1330,15 +1394,20 @@ extern "C" int mv_herk_grouped(int form, int nterms, int nprod, int tile,
   MV_REQUIRE(total_tiles < 65536 && max_items < (int64_t)INT32_MAX / 2,
              "mv_herk_grouped: too many tiles (%lld) or items",
              (long long)total_tiles);
-  const GroupLayout g = group_layout(tile, problems_host, nprob);
+  const GroupLayout g = group_layout(form, tile, problems_host, nprob);
   MV_REQUIRE(ws_bytes >= (size_t)g.total,
              "mv_herk_grouped: workspace too small (%zu < %lld)", ws_bytes,
              (long long)g.total);
   const bool f16 = form == MV_FORM_SPLIT_F16;
-  const int threads = tile == 64 ? SplitShape<64>::kThreads
-                                 : SplitShape<kST>::kThreads;
-  const int slots = kernel_slots(grouped_kernel(tile, f16, nterms, maxord),
-                                 threads);
+  const void *kernel =
+      split ? grouped_kernel(tile, f16, nterms, maxord)
+            : (form == MV_FORM_C128
+                   ? (const void *)&herk_plain_grouped_kernel<double>
+                   : (const void *)&herk_plain_grouped_kernel<float>);
+  const int threads = !split ? 256
+                      : tile == 64 ? SplitShape<64>::kThreads
+                                   : SplitShape<kST>::kThreads;
+  const int slots = kernel_slots(kernel, threads);
   if (slots <= 0) {
     set_error(MV_ERR_HIP, "mv_herk_grouped: occupancy query failed");
     return MV_ERR_HIP;
@@ -1348,11 +1417,32 @@ extern "C" int mv_herk_grouped(int form, int nterms, int nprod, int tile,
   GroupPlan plan{(int32_t *)(ws + g.ns), (int64_t *)(ws + g.item_off),
                  (int32_t *)(ws + g.tile_off)};
   hipStream_t st = as_stream(stream);
-  herk_group_plan_kernel<<<1, 256, 0, st>>>(problems_dev, nprob, tile, kSK,
-                                            form_cmax(form), probs,
-                                            ws + g.parts, plan);
+  // slices of a problem: split forms bound the FP32 chain (form_cmax);
+  // plain forms take slices of 64 K tiles (1024 columns) for parallelism
+  herk_group_plan_kernel<<<1, 256, 0, st>>>(
+      problems_dev, nprob, tile, split ? kSK : kPK,
+      split ? form_cmax(form) : 64, part_bytes(form), split ? 1 : 0, probs,
+      ws + g.parts, plan);
   if (int e = check_launch("mv_herk_grouped/plan")) return e;
   const int grid = (int)(max_items < slots ? max_items : slots);
+  const dim3 rgrid((unsigned)cdiv((int64_t)tile * tile / 2, (int64_t)256),
+                   (unsigned)total_tiles);
+  if (!split) {
+    if (form == MV_FORM_C128) {
+      herk_plain_grouped_kernel<double><<<grid, 256, 0, st>>>(probs, nprob,
+                                                              plan);
+      if (int e = check_launch("mv_herk_grouped/plain")) return e;
+      herk_group_reduce_kernel<double><<<rgrid, 256, 0, st>>>(probs, nprob,
+                                                             tile, plan);
+    } else {
+      herk_plain_grouped_kernel<float><<<grid, 256, 0, st>>>(probs, nprob,
+                                                             plan);
+      if (int e = check_launch("mv_herk_grouped/plain")) return e;
+      herk_group_reduce_kernel<float><<<rgrid, 256, 0, st>>>(probs, nprob,
+                                                            tile, plan);
+    }
+    return check_launch("mv_herk_grouped/reduce");
+  }
 #define MV_GROUP_LAUNCH(NT, MO, F, T)                                        \
   herk_split_grouped_kernel<NT, MO, F, T>                                    \
       <<<grid, SplitShape<T>::kThreads, 0, st>>>(probs, nprob, plan)
@@ -1375,10 +1465,8 @@ extern "C" int mv_herk_grouped(int form, int nterms, int nprod, int tile,
 #undef MV_GROUP_TILE
 #undef MV_GROUP_LAUNCH
   if (int e = check_launch("mv_herk_grouped/split")) return e;
-  herk_group_reduce_kernel<<<dim3((unsigned)cdiv((int64_t)tile * tile / 2,
-                                                 (int64_t)256),
-                                  (unsigned)total_tiles),
-                             256, 0, st>>>(probs, nprob, tile, plan);
+  herk_group_reduce_kernel<float><<<rgrid, 256, 0, st>>>(probs, nprob, tile,
+                                                        plan);
   return check_launch("mv_herk_grouped/reduce");
 }
 

@@ -1,6 +1,7 @@
-"""Grouped split HERK: many instances of different sizes in one persistent
-launch (``mv_herk_grouped``), optionally captured once in a HIP graph and
-replayed.
+"""Grouped HERK: many instances of one form and different sizes in one
+persistent launch (``mv_herk_grouped``), optionally captured once in a HIP
+graph and replayed.  Split forms use 64- or 128-row tiles, the plain forms
+(complex128, complex64) 64-row tiles.
 
 A :class:`HerkGroup` holds one tile class (64 or 128 rows per workgroup
 tile) of a batch: the packed problem descriptors on the host and the device,
@@ -18,6 +19,22 @@ from .. import _device as D
 from .._lib import call, lib
 from .herk import Herk, is_split
 
+
+def plain_tile() -> int:
+    """Tile side of the plain forms' grouped launch."""
+    from .._lib import FORM_C128
+
+    return int(lib().mv_herk_tile(FORM_C128))
+
+
+def tile_for(form: int, m: int) -> int:
+    """Default tile of one problem: 64 rows for small split problems, the
+    form's own tile otherwise."""
+    if not is_split(form):
+        return plain_tile()
+    return 64 if m <= 128 else 128
+
+
 #: Layout of ``mv_herk_problem`` (include/matvis_hip.h), 96 bytes.
 PROBLEM_DTYPE = np.dtype([
     ("z", "<u8", (3,)), ("ldz", "<i8"), ("seg_stride", "<i8"),
@@ -32,9 +49,10 @@ def _addr(t) -> int:
 
 
 class HerkGroup:
-    """One grouped launch over problems sharing a split form and a tile.
+    """One grouped launch over problems sharing a form and a tile.
 
-    ``problems``: list of dicts with ``terms`` (list of int32 term tensors),
+    ``problems``: list of dicts with ``terms`` (list of int32 term tensors
+    for split forms, one complex tensor for the plain forms),
     ``m``, ``ldz``, ``nseg``, ``seg_stride``, ``seg_len`` (host bound),
     optional ``seg_len_dev`` (device int32 tensor), ``vacc`` (complex128
     tensor, at least ldv x ldv), ``ldv``, ``scale``, ``accumulate``.
@@ -43,8 +61,6 @@ class HerkGroup:
 
     def __init__(self, form: int, nterms: int, nprod: int, tile: int,
                  problems: list):
-        if not is_split(form):
-            raise ValueError("grouped HERK runs split forms only")
         if lib().mv_herk_problem_bytes() != PROBLEM_DTYPE.itemsize:
             raise RuntimeError("mv_herk_problem layout mismatch")
         self.form, self.nterms, self.nprod, self.tile = form, nterms, nprod, tile
@@ -97,7 +113,7 @@ class HerkGroup:
 
 
 class GroupedBatch:
-    """A batch of split HERK problems, one grouped launch per tile class;
+    """A batch of HERK problems of one form, one grouped launch per tile class;
     ``capture()`` records the launches in a HIP graph for ``replay()``."""
 
     def __init__(self, form: int, nterms: int, nprod: int, problems: list,

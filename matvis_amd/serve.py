@@ -7,12 +7,12 @@ HBM budget; admitted requests stay resident (their engine keeps antenna
 positions, sky, beam tables and buffers on the device) until their last
 (time, frequency) instance is done.  Each scheduler step takes the next
 instance of every admitted request: the per-time rotation and cull, the beam
-and the fused Z kernel run per request; the HERKs of all split-BF16 /
-split-FP16 instances of the step then run as one grouped persistent launch
-(captured in a HIP graph the first time a set of requests steps together,
-replayed while that set lasts), FP64 / FP32 instances as their own plain
-launches.  Requests that arrive while others run are admitted at the next
-step, as memory frees up.
+and the fused Z kernel run per request; the HERKs of all instances of the
+step then run as grouped persistent launches, one per HERK form present
+(split-BF16, split-FP16, complex128, complex64), each captured in a HIP graph
+the first time a set of requests steps together and replayed while that set
+lasts.  Requests that arrive while others run are admitted at the next step,
+as memory frees up.
 
 Every instance's HERK depends only on its own data (the slice count comes
 from its own column count), so a request's result is the same alone and in
@@ -238,10 +238,8 @@ class VisibilityServer:
                 (split_probs if a.split else plain).append((a, prob))
             except Exception as exc:  # reported on the request
                 failed.append((a, exc))
-        for a, prob in plain:
-            a.engine.run_herk(prob)
-        if split_probs:
-            self._grouped(split_probs)
+        if split_probs or plain:
+            self._grouped(split_probs + plain)
         done = []
         for a in self.active:
             if any(a is f for f, _ in failed):
@@ -351,11 +349,9 @@ class VisibilityServer:
                    / (1.0 + (now - self.queue[i].t_submit) / self.age_s))
 
     def _grouped(self, items):
-        from .gpu.herk_group import GroupedBatch
+        from .gpu.herk_group import GroupedBatch, tile_for
 
-        first = items[0][0].engine
-        form, nterms, nprod = first.form, first.nterms, first.nprod
-        # one grouped launch per split variant present in the step
+        # one grouped launch per form variant present in the step
         variants: dict[tuple, list] = {}
         for a, prob in items:
             e = a.engine
@@ -365,7 +361,7 @@ class VisibilityServer:
             probs = [p for _, p in its]
             batch = self._batches.get(key)
             if batch is None:
-                tiles = [64 if p["m"] <= 128 else 128 for p in probs]
+                tiles = [tile_for(form, p["m"]) for p in probs]
                 batch = GroupedBatch(form, nterms, nprod, probs, tiles)
                 if self.graphs:
                     batch.capture()

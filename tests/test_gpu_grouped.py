@@ -110,3 +110,47 @@ def test_grouped_graph_replays():
         torch.cuda.synchronize()
         for p, w in zip(probs, want):
             assert torch.equal(p["vacc"], w), f"replay {rep} differs"
+
+
+
+@pytest.mark.parametrize("form", ["c128", "c64"])
+def test_grouped_plain_matches_single_launches(form):
+    """Plain forms in one grouped launch (64-row tiles) against one plain
+    launch per problem: FP64 to rounding, complex64 to FP32 accumulation."""
+    from matvis_amd._lib import FORM_C64, FORM_C128
+    from matvis_amd.gpu.herk import Herk
+    from matvis_amd.gpu.herk_group import HerkGroup, plain_tile
+
+    f = FORM_C128 if form == "c128" else FORM_C64
+    ct = np.complex128 if form == "c128" else np.complex64
+    print(f"grouped plain {form}", flush=True)
+    probs, singles = [], []
+    for i, (m, seg_len, nseg, n_dev) in enumerate(CASES):
+        rng = np.random.default_rng(40 + i)
+        seg = -(-seg_len // 16) * 16
+        z = (rng.normal(size=(m, nseg * seg))
+             + 1j * rng.normal(size=(m, nseg * seg))).astype(ct)
+        n = seg_len if n_dev is None else n_dev
+        for s_ in range(nseg):
+            z[:, s_ * seg + n:(s_ + 1) * seg] = 0
+        zd = D.to_dev(z)
+        h = Herk(f, m)
+        probs.append(dict(terms=[zd], m=m, ldz=nseg * seg, nseg=nseg,
+                          seg_stride=seg, seg_len=seg_len,
+                          seg_len_dev=None if n_dev is None else D.to_dev(
+                              np.array([n_dev], np.int32)),
+                          vacc=h.new_vacc(), ldv=h.ldv, scale=1.0,
+                          accumulate=False))
+        v = h.new_vacc()
+        h.run([zd], ldz=nseg * seg, nseg=nseg, seg_stride=seg,
+              seg_len=seg_len, seg_len_dev=probs[-1]["seg_len_dev"], vacc=v,
+              accumulate=False, nslices=h.nslices(nseg * seg_len))
+        singles.append((h, v))
+    g = HerkGroup(f, 1, 1, plain_tile(), probs)
+    g.run()
+    for p, (h, v), c in zip(probs, singles, CASES):
+        want = _full(v, h.ldv, p["m"], h.tile)
+        got = _full(p["vacc"], p["ldv"], p["m"], plain_tile())
+        e = _nerr(got, want)
+        print(f"  case {c}: grouped vs single {e:.2e}", flush=True)
+        assert e < (1e-13 if form == "c128" else 2e-6), (c, e)

@@ -63,18 +63,17 @@ def test_server_matches_simulate_multi():
     _, got = _serve(MIX)
     for s, g in zip(MIX, got):
         want = simulate_multi(**_kwargs(*s))
-        if s[5] == "GPUMatMul":
-            assert np.array_equal(g, want), s  # same kernels, same launches
-        else:
-            # grouped vs single-problem split launches: different K-slicing
-            import oracle as O
-            nf = 2 if s[4] else 1
-            for fi in range(g.shape[0]):
-                a, b = g[fi], want[fi]
-                if not s[4]:
-                    a, b = a[..., None, None], b[..., None, None]
-                e = O.normalized_error(a, b, s[0], nf)[0]
-                assert e < 1e-6, (s, fi, e)
+        # grouped vs single-problem launches slice K differently: FP64
+        # agrees to rounding, the split forms to their accumulation error
+        import oracle as O
+        nf = 2 if s[4] else 1
+        bound = 1e-12 if s[5] == "GPUMatMul" else 1e-6
+        for fi in range(g.shape[0]):
+            a, b = g[fi], want[fi]
+            if not s[4]:
+                a, b = a[..., None, None], b[..., None, None]
+            e = O.normalized_error(a, b, s[0], nf)[0]
+            assert e < bound, (s, fi, e)
 
 
 def test_admission_within_budget():
