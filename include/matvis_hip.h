@@ -106,6 +106,29 @@ int mv_bcrs_update(const double *eci, int64_t ld_eci, int64_t nsrc,
                    const double *ctx17, double dlim, double *out,
                    int64_t ld_out, mv_stream_t stream);
 
+/* Rotation of polarized sources' coherency matrices from the equatorial
+ * (RA/Dec) basis into the alt/az basis, for the sources mv_rotate_cull kept.
+ * Replaces the polarized branch of CoordinateRotation.select_chunk with
+ * _rotate_frame_coherency (src/matvis/core/coords.py:123-175) and
+ * calc_coherency_rotation (src/matvis/coordinates.py:288-491).
+ * ra, dec: f64 (nsrc) radians, by global source id.  idx, crd, n_up_dev: the
+ * idx_out (int32, nalloc), crd_out ((3, ld_crd), f64 if crd_is_f64 else f32)
+ * and count mv_rotate_cull wrote.  r_avg9 (host): row-major 3x3 carrying ICRS
+ * unit vectors to the (north, east, up) frame at the time.
+ * coh: (nsrc, nfreq, 2, 2), out: (nalloc, nfreq, 2, 2), both f64 if
+ * coh_is_f64 else f32, complex (interleaved re, im) if coh_is_complex.
+ * out[s] = rot^T coh[idx[s]] rot with rot = [[cos X, sin X], [-sin X,
+ * cos X]] per source; positions [n_up, nalloc) are zero-filled.  Computed
+ * in FP64 throughout; where the per-source alignment rotation's axis is
+ * undefined (the two directions coincide) it is the identity, the limit the
+ * reference's 0/0 leaves as NaN. */
+int mv_coherency_rotate(const double *ra, const double *dec, int64_t nsrc,
+                        const int32_t *idx, const void *crd, int64_t ld_crd,
+                        int crd_is_f64, const int32_t *n_up_dev,
+                        int64_t nalloc, const double *r_avg9, const void *coh,
+                        int64_t nfreq, int coh_is_f64, int coh_is_complex,
+                        void *out, mv_stream_t stream);
+
 /* ---------------------------------------------------------------- (b) ---- */
 
 /* In-place cubic B-spline prefilter of beam planes (mirror boundary), the

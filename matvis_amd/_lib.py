@@ -38,6 +38,8 @@ SIGNATURES: dict[str, list] = {
     "mv_rotate_cull": [_P, _L, _P, _P, _L, _L, _L, _P, _L, _P, _I, _P, _P,
                        _P, _S, _P],
     "mv_bcrs_update": [_P, _L, _L, _P, _D, _P, _L, _P],
+    "mv_coherency_rotate": [_P, _P, _L, _P, _P, _L, _I, _P, _L, _P, _P, _L,
+                            _I, _I, _P, _P],
     "mv_beam_prefilter": [_P, _I, _L, _I, _I, _P],
     "mv_beam_interp": [_P, _I, _I, _I, _I, _I, _I, _P, _I, _I, _P, _I, _L,
                        _L, _P, _P, _L, _P],

@@ -8,6 +8,12 @@ above-horizon coordinates, their fluxes (zero past ``n``) and ``n``.
 Subclasses provide the per-time rotation either as a 3x3 matrix R_t
 (:meth:`rotation_matrix`; the device applies it fused with the cull) or by
 filling topocentric vectors themselves (:meth:`set_topocentric`).
+
+A polarized sky model is a flux of shape (nsrc, nfreq, 2, 2), real or
+complex: the coherency matrix of each source per frequency in the RA/Dec
+basis.  ``select_chunk`` then returns the kept sources' coherency matrices
+rotated into the alt/az basis (``mv_coherency_rotate``), which needs the
+time's average ICRS -> (north, east, up) rotation (:meth:`average_rotation`).
 """
 
 from __future__ import annotations
@@ -54,14 +60,21 @@ class CoordinateRotation(ABC):
         self.precision = precision
         self.rtype, self.ctype = get_dtypes(precision)
         flux = np.asarray(flux)
-        if np.iscomplexobj(flux) or flux.ndim != 1:
+        self._polarized = flux.ndim == 4
+        if self._polarized:
+            if flux.shape[2:] != (2, 2):
+                raise ValueError(
+                    "a polarized flux must have shape (NSRCS, NFREQ, 2, 2)")
+        elif flux.ndim != 1 or np.iscomplexobj(flux):
             raise NotImplementedError(
-                "polarized (coherency-matrix) sky models are not supported; "
-                "I_sky must be real with shape (NSRCS,)"
+                "flux must be real with shape (NSRCS,), or a coherency "
+                "matrix per source and frequency, (NSRCS, NFREQ, 2, 2)"
             )
-        self.sky_model_dtype = self.rtype
-        self.flux = flux.astype(np.float64)
-        self._polarized = False
+        # dtype rule of core/coords.py:59-63
+        self.sky_model_dtype = (self.ctype if np.iscomplexobj(flux)
+                                else self.rtype)
+        self.flux = (flux.astype(self.sky_model_dtype) if self._polarized
+                     else flux.astype(np.float64))
         self.nsrc = len(flux)
         self.times = times
         self.telescope_loc = telescope_loc
@@ -89,11 +102,19 @@ class CoordinateRotation(ABC):
         """Allocate device buffers and upload source vectors and fluxes."""
         ra, dec = skycoord_ra_dec(self.skycoords)
         self.eci = D.to_dev(point_source_crd_eq(ra, dec), np.float64)
-        self.flux_dev = D.to_dev(self.flux, np.float64)
         self._topo = D.zeros((3, self.nsrc), self.rtype)
         self._topo64 = None
         self.coords_above_horizon = D.zeros((3, self.nsrc_alloc), self.rtype)
-        self.flux_above_horizon = D.zeros((self.nsrc_alloc,), self.rtype)
+        if self._polarized:
+            self.flux_dev = D.to_dev(self.flux)
+            self.flux_above_horizon = D.zeros(
+                (self.nsrc_alloc,) + self.flux.shape[1:], self.sky_model_dtype)
+            self._ra = D.to_dev(np.asarray(ra, np.float64))
+            self._dec = D.to_dev(np.asarray(dec, np.float64))
+            self._idx = D.zeros((self.nsrc_alloc,), np.int32)
+        else:
+            self.flux_dev = D.to_dev(self.flux, np.float64)
+            self.flux_above_horizon = D.zeros((self.nsrc_alloc,), self.rtype)
         self.n_up_dev = D.zeros((1,), np.int32)
         nbytes = lib().mv_rotate_cull_workspace_bytes(self.chunk_size)
         self.workspace = D.empty((nbytes,), np.uint8)
@@ -116,6 +137,19 @@ class CoordinateRotation(ABC):
         self._rot = np.ascontiguousarray(rot, dtype=np.float64)
         self._mode = "eci"
         self._topo_valid = False
+
+    def average_rotation(self, t: int) -> np.ndarray:
+        """3x3 rotation carrying ICRS unit vectors to the (north, east, up)
+        frame at time index t: the role of ``_calc_average_rotation_matrix``
+        (coordinates.py:493-534).  Here R_t with its first two rows swapped;
+        subclasses without an R_t override it."""
+        rot = self.rotation_matrix(t)
+        if rot is None:
+            raise NotImplementedError(
+                f"{type(self).__name__} must implement average_rotation() "
+                "for polarized sky models"
+            )
+        return np.ascontiguousarray(np.asarray(rot, np.float64)[[1, 0, 2]])
 
     def set_topocentric(self, enu: np.ndarray) -> None:
         """Use host-computed ENU unit vectors (3, nsrc) for the current time."""
@@ -158,14 +192,28 @@ class CoordinateRotation(ABC):
         chunk0 = chunk * self.chunk_size
         chunk_len = max(0, min(self.chunk_size, self.nsrc - chunk0))
         chunk0 = min(chunk0, self.nsrc)
+        pol = self._polarized
+        f64 = int(self.precision == 2)
         call(
             "mv_rotate_cull", D.ptr(src), self.nsrc, self._rot.ctypes.data,
-            D.ptr(self.flux_dev), chunk0, chunk_len, self.nsrc_alloc,
-            D.ptr(self.coords_above_horizon), self.nsrc_alloc,
-            D.ptr(self.flux_above_horizon), int(self.precision == 2), None,
-            D.ptr(self.n_up_dev), D.ptr(self.workspace),
-            self.workspace.numel(), D.stream_ptr(),
+            None if pol else D.ptr(self.flux_dev), chunk0, chunk_len,
+            self.nsrc_alloc, D.ptr(self.coords_above_horizon), self.nsrc_alloc,
+            None if pol else D.ptr(self.flux_above_horizon), f64,
+            D.ptr(self._idx) if pol else None, D.ptr(self.n_up_dev),
+            D.ptr(self.workspace), self.workspace.numel(), D.stream_ptr(),
         )
+        if pol:
+            r_avg = np.ascontiguousarray(self.average_rotation(t), np.float64)
+            if r_avg.shape != (3, 3):
+                raise ValueError("average_rotation() must return a 3x3 matrix")
+            call(
+                "mv_coherency_rotate", D.ptr(self._ra), D.ptr(self._dec),
+                self.nsrc, D.ptr(self._idx), D.ptr(self.coords_above_horizon),
+                self.nsrc_alloc, f64, D.ptr(self.n_up_dev), self.nsrc_alloc,
+                r_avg.ctypes.data, D.ptr(self.flux_dev), self.flux.shape[1],
+                f64, int(np.iscomplexobj(self.flux)),
+                D.ptr(self.flux_above_horizon), D.stream_ptr(),
+            )
         return self.coords_above_horizon, self.flux_above_horizon, self.n_up_dev
 
     def select_chunk(self, chunk: int, t: int):

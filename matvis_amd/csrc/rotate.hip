@@ -322,3 +322,167 @@ extern "C" int mv_rotate_cull(const double *eci, int64_t ld_eci,
         (float *)crd_out, ld_crd, (float *)flux_out, idx_out, nb);
   return check_launch("mv_rotate_cull/scatter");
 }
+
+// ------------------------------------------------- coherency (polarized sky)
+
+namespace mv {
+namespace {
+
+// Angle X between the equatorial basis (theta_hat, phi_hat at colatitude
+// pi/2 - dec, longitude ra) and the alt/az basis (at za, az) of one source,
+// returned as (cos X, sin X).  R_avg carries ICRS unit vectors to the
+// (north, east, up) frame; the small rotation taking R_avg . frame onto the
+// source's alt/az direction (Rodrigues about their cross product) completes
+// it for the source (src/matvis/coordinates.py:288-491).  Parallel vectors
+// give the identity, the limit of the rotation as its angle goes to zero.
+__device__ void basis_angle(double ra, double dec, double e, double n,
+                            const Mat3 &R, double &cx, double &sx) {
+  // az / za from the direction cosines as enu_to_az_za (coordinates.py:31-88)
+  const double lsqr = n * n + e * e;
+  const double zeta = lsqr < 1.0 ? sqrt(1.0 - lsqr) : 0.0;
+  const double az = atan2(e, n);
+  const double za = 0.5 * kPi - asin(zeta);
+  const double alt = 0.5 * kPi - za;
+  double saz, caz, salt, calt, sra, cra, sdec, cdec;
+  sincos(az, &saz, &caz);
+  sincos(alt, &salt, &calt);
+  sincos(ra, &sra, &cra);
+  sincos(dec, &sdec, &cdec);
+  const double f[3] = {cra * cdec, cdec * sra, sdec};      // ICRS direction
+  const double h[3] = {caz * calt, calt * saz, salt};      // (N, E, U)
+  double p[3];
+  for (int i = 0; i < 3; ++i)
+    p[i] = R.m[3 * i] * f[0] + R.m[3 * i + 1] * f[1] + R.m[3 * i + 2] * f[2];
+  const double c3[3] = {p[1] * h[2] - p[2] * h[1], p[2] * h[0] - p[0] * h[2],
+                        p[0] * h[1] - p[1] * h[0]};
+  const double sp = sqrt(c3[0] * c3[0] + c3[1] * c3[1] + c3[2] * c3[2]);
+  const double cp = p[0] * h[0] + p[1] * h[1] + p[2] * h[2];
+  // perturbation P = I + sin(psi) K + (1 - cos(psi)) K^2, K = [axis]_x
+  double P[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  if (sp > 0.0) {
+    const double ax = c3[0] / sp, ay = c3[1] / sp, az3 = c3[2] / sp;
+    const double psi = atan2(sp, cp);
+    double s, c;
+    sincos(psi, &s, &c);
+    const double K[9] = {0, -az3, ay, az3, 0, -ax, -ay, ax, 0};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double k2 = 0.0;
+        for (int q = 0; q < 3; ++q) k2 += K[3 * i + q] * K[3 * q + j];
+        P[3 * i + j] += s * K[3 * i + j] + (1.0 - c) * k2;
+      }
+  }
+  // theta_hat of the alt/az basis at (colatitude za, longitude az), carried
+  // back to ICRS by (P R_avg)^T
+  double sbe, cbe;
+  sincos(0.5 * kPi - alt, &sbe, &cbe);
+  const double th[3] = {caz * cbe, saz * cbe, -sbe};
+  double q[3];
+  for (int j = 0; j < 3; ++j)
+    q[j] = P[j] * th[0] + P[3 + j] * th[1] + P[6 + j] * th[2];  // P^T th
+  double b[3];
+  for (int j = 0; j < 3; ++j)
+    b[j] = R.m[j] * q[0] + R.m[3 + j] * q[1] + R.m[6 + j] * q[2];  // R^T q
+  double sth, cth;
+  sincos(0.5 * kPi - dec, &sth, &cth);
+  cx = b[0] * cra * cth + b[1] * sra * cth - b[2] * sth;   // . theta_hat
+  sx = -b[0] * sra + b[1] * cra;                          // . phi_hat
+}
+
+// C' = rot^T C rot with rot = [[c, s], [-s, c]], for real or complex
+// (re, im pairs) entries of one 2x2 coherency matrix.
+template <typename T, int NC>
+__device__ __forceinline__ void rotate_2x2(const T *in, T *out, double c,
+                                           double s) {
+  for (int k = 0; k < NC; ++k) {
+    const double a = in[0 * NC + k], b = in[1 * NC + k];
+    const double d = in[2 * NC + k], e = in[3 * NC + k];
+    // M = rot^T C
+    const double m00 = c * a - s * d, m01 = c * b - s * e;
+    const double m10 = s * a + c * d, m11 = s * b + c * e;
+    out[0 * NC + k] = (T)(m00 * c - m01 * s);
+    out[1 * NC + k] = (T)(m00 * s + m01 * c);
+    out[2 * NC + k] = (T)(m10 * c - m11 * s);
+    out[3 * NC + k] = (T)(m10 * s + m11 * c);
+  }
+}
+
+// One thread per output position s: the kept source idx[s] (s < n_up) has
+// its (nfreq, 2, 2) coherency rotated; positions [n_up, nalloc) are zeroed,
+// as is a position whose index lies outside [0, nsrc).
+template <typename T, typename C, int NC>
+__global__ __launch_bounds__(kThreads) void coherency_kernel(
+    const double *__restrict__ ra, const double *__restrict__ dec,
+    int64_t nsrc, const int32_t *__restrict__ idx, const C *__restrict__ crd,
+    int64_t ld_crd, const int32_t *__restrict__ n_up_dev, int64_t nalloc,
+    Mat3 R, const T *__restrict__ coh, int64_t nfreq, T *__restrict__ out) {
+  const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (s >= nalloc) return;
+  const int64_t per = nfreq * 4 * NC;  // scalars per source
+  T *o = out + s * per;
+  const int64_t n_up = *n_up_dev;
+  const int64_t g = s < n_up ? idx[s] : -1;
+  if (g < 0 || g >= nsrc) {
+    for (int64_t q = 0; q < per; ++q) o[q] = T(0);
+    return;
+  }
+  double cx, sx;
+  basis_angle(ra[g], dec[g], (double)crd[s], (double)crd[ld_crd + s], R, cx,
+              sx);
+  const T *in = coh + g * per;
+  for (int64_t f = 0; f < nfreq; ++f)
+    rotate_2x2<T, NC>(in + f * 4 * NC, o + f * 4 * NC, cx, sx);
+}
+
+template <typename C>
+int launch_coherency(const double *ra, const double *dec, int64_t nsrc,
+                     const int32_t *idx, const C *crd, int64_t ld_crd, const int32_t *n_up_dev,
+                     int64_t nalloc, const Mat3 &R, const void *coh,
+                     int64_t nfreq, int is_f64, int is_complex, void *out,
+                     hipStream_t s) {
+  const dim3 grid((unsigned)cdiv(nalloc, kThreads));
+#define MV_COH(T, NC)                                                        \
+  coherency_kernel<T, C, NC><<<grid, kThreads, 0, s>>>(                      \
+      ra, dec, nsrc, idx, crd, ld_crd, n_up_dev, nalloc, R, (const T *)coh,  \
+      nfreq, (T *)out)
+  if (is_f64 && is_complex) MV_COH(double, 2);
+  else if (is_f64) MV_COH(double, 1);
+  else if (is_complex) MV_COH(float, 2);
+  else MV_COH(float, 1);
+#undef MV_COH
+  return check_launch("mv_coherency_rotate");
+}
+
+}  // namespace
+}  // namespace mv
+
+extern "C" int mv_coherency_rotate(const double *ra, const double *dec,
+                                   int64_t nsrc, const int32_t *idx,
+                                   const void *crd, int64_t ld_crd,
+                                   int crd_is_f64,
+                                   const int32_t *n_up_dev, int64_t nalloc,
+                                   const double *r_avg9, const void *coh,
+                                   int64_t nfreq, int coh_is_f64,
+                                   int coh_is_complex, void *out,
+                                   mv_stream_t stream) {
+  using namespace mv;
+  MV_REQUIRE(ra && dec && crd && n_up_dev && r_avg9 && out,
+             "mv_coherency_rotate: null pointer");
+  MV_REQUIRE(nsrc >= 0 && nalloc >= 0 && ld_crd >= nalloc && nfreq >= 1,
+             "mv_coherency_rotate: bad sizes nsrc=%lld nalloc=%lld "
+             "ld_crd=%lld nfreq=%lld",
+             (long long)nsrc, (long long)nalloc, (long long)ld_crd,
+             (long long)nfreq);
+  MV_REQUIRE(nalloc == 0 || (idx && coh),
+             "mv_coherency_rotate: null idx or coherency");
+  if (nalloc == 0) return MV_OK;
+  const Mat3 R = load_mat(r_avg9);
+  hipStream_t s = as_stream(stream);
+  return crd_is_f64
+             ? launch_coherency(ra, dec, nsrc, idx, (const double *)crd,
+                                ld_crd, n_up_dev, nalloc, R, coh, nfreq, coh_is_f64,
+                                coh_is_complex, out, s)
+             : launch_coherency(ra, dec, nsrc, idx, (const float *)crd,
+                                ld_crd, n_up_dev, nalloc, R, coh, nfreq, coh_is_f64,
+                                coh_is_complex, out, s);
+}

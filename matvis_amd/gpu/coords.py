@@ -120,6 +120,16 @@ class CoordinateRotationERFA(CoordinateRotation):
         self._mode = "bcrs"
         self._topo_valid = False
 
+    def average_rotation(self, t: int) -> np.ndarray:
+        """ICRS -> (north, east, up) without the per-source deflection and
+        aberration: the CIRS -> ENU matrix times bias-precession-nutation,
+        first two rows swapped.  The per-source remainder (up to the ~20
+        arcsec of aberration) is the alignment rotation that
+        ``mv_coherency_rotate`` adds per source."""
+        ctx = self.context(t)
+        full = atioq_matrix(ctx) @ np.asarray(ctx["bpn"], np.float64)
+        return np.ascontiguousarray(full[[1, 0, 2]])
+
 
 class GPUCoordinateRotationERFA(CoordinateRotationERFA):
     """The name the reference's GPU path registers (gpu/coords.py:38-53);
@@ -172,6 +182,22 @@ class CoordinateRotationAstropy(CoordinateRotation):
             np.array([np.cos(el) * np.sin(az), np.cos(el) * np.cos(az),
                       np.sin(el)])
         )
+
+    def average_rotation(self, t: int) -> np.ndarray:
+        """The ICRS unit axes transformed to AltAz at time t, as Cartesian
+        (north, east, up) columns, made exactly orthogonal by the polar
+        factor of their SVD (coordinates.py:493-534)."""
+        from astropy.coordinates import AltAz, SkyCoord
+
+        axes = SkyCoord(x=[1.0, 0.0, 0.0], y=[0.0, 1.0, 0.0],
+                        z=[0.0, 0.0, 1.0], frame="icrs",
+                        representation_type="cartesian")
+        frame = AltAz(obstime=self.times[t], location=self.telescope_loc)
+        altaz = axes.transform_to(frame)
+        altaz.representation_type = "cartesian"
+        m = np.asarray(altaz.cartesian.xyz.value, np.float64)
+        u, _, vt = np.linalg.svd(m)
+        return np.ascontiguousarray(u @ vt)
 
 
 def source_vectors(skycoords) -> np.ndarray:

@@ -53,6 +53,15 @@ Parity status per function (see ``tests/test_oracle.py``):
 ``erfa_bcrs``,              pinned: pyerfa's eraAtco13 observed directions; the
 ``erfa_atioq_matrix``       residual equals the diurnal aberration the
                             reference omits (checked by applying it)
+``vecs2rot``,               pinned: carries r1 onto r2, orthonormal with
+``axis_angle_rotation``     determinant 1, quarter turns about the axes
+``coherency_rotation``      pinned: for a rigid Earth the angle X is minus
+                            the textbook parallactic angle
+                            atan2(sin H, tan(lat) cos(dec) - sin(dec) cos H)
+``rotate_coherency``        pinned: Stokes I, V and Q^2 + U^2 invariant;
+                            Q' = Q cos 2X - U sin 2X, U' = Q sin 2X + U cos 2X
+``select_chunk_polarized``  pinned: the two above plus the compaction of
+                            ``select_chunk``
 ==========================  =====================================================
 """
 
@@ -163,6 +172,95 @@ def enu_to_az_za(enu_e, enu_n, orientation="astropy", periodic_azimuth=True):
     if periodic_azimuth:
         az = np.mod(az, 2 * np.pi)
     return az, za
+
+
+def theta_hat(theta, phi):
+    """Unit vector of increasing colatitude at (theta, phi)
+    (coordinates.py:537-556)."""
+    theta, phi = np.asarray(theta, np.float64), np.asarray(phi, np.float64)
+    return np.array([np.cos(phi) * np.cos(theta), np.sin(phi) * np.cos(theta),
+                     -np.sin(theta)])
+
+
+def phi_hat(theta, phi):
+    """Unit vector of increasing longitude at (theta, phi)
+    (coordinates.py:559-576)."""
+    phi = np.asarray(phi, np.float64)
+    return np.array([-np.sin(phi), np.cos(phi), np.zeros_like(phi)])
+
+
+def axis_angle_rotation(axis, angle):
+    """Rodrigues: I + sin(a) K + (1 - cos(a)) K^2 with K the cross-product
+    matrix of each unit axis; (3, 3, N) (coordinates.py:414-452)."""
+    axis = np.asarray(axis, np.float64)
+    z = np.zeros(axis.shape[1])
+    k = np.array([[z, -axis[2], axis[1]],
+                  [axis[2], z, -axis[0]],
+                  [-axis[1], axis[0], z]])
+    k2 = np.einsum("ijn,jkn->ikn", k, k)
+    return np.eye(3)[..., None] + np.sin(angle) * k + (1.0 - np.cos(angle)) * k2
+
+
+def vecs2rot(r1, r2):
+    """Rotation about r1 x r2 by the angle between them, carrying each r1 to
+    r2 (coordinates.py:387-411).  Where r1 x r2 vanishes the reference's
+    axis is 0/0 (NaN); here it is the identity, the limit as the angle goes
+    to zero."""
+    cross = np.cross(r1, r2, axis=0)
+    sin_psi = np.linalg.norm(cross, axis=0)
+    cos_psi = np.sum(r1 * r2, axis=0)
+    safe = np.where(sin_psi > 0, sin_psi, 1.0)
+    axis = cross / safe
+    psi = np.where(sin_psi > 0, np.arctan2(sin_psi, cos_psi), 0.0)
+    return axis_angle_rotation(axis, psi)
+
+
+def coherency_rotation(ra, dec, alt, az, r_avg):
+    """2x2 rotation (2, 2, N) carrying a coherency matrix from the RA/Dec
+    basis to the alt/az basis (coordinates.py:288-384, 455-490): the average
+    ICRS -> (N, E, U) rotation ``r_avg`` (3x3) composed with the per-source
+    rotation that lands it on the source's (alt, az) direction; X is the
+    angle between the equatorial theta_hat / phi_hat and the alt/az
+    theta_hat carried back to ICRS."""
+    frame = point_source_crd_eq(ra, dec)
+    altaz = point_source_crd_eq(az, alt)
+    r_avg = np.asarray(r_avg, np.float64)
+    perturb = vecs2rot(r_avg @ frame, altaz)
+    exact = np.einsum("abn,bc->acn", perturb, r_avg)
+    theta, phi = 0.5 * np.pi - np.asarray(dec), np.asarray(ra)
+    bh = np.einsum("ban,bn->an", exact,
+                   theta_hat(0.5 * np.pi - np.asarray(alt), az))
+    cos_x = np.einsum("an,an->n", bh, theta_hat(theta, phi))
+    sin_x = np.einsum("an,an->n", bh, phi_hat(theta, phi))
+    return np.array([[cos_x, sin_x], [-sin_x, cos_x]])
+
+
+def rotate_coherency(coh, rot):
+    """rot^T C rot per source and frequency: coh (N, nfreq, 2, 2), rot
+    (2, 2, N) (core/coords.py:169-175)."""
+    return np.einsum("ban,nfbc,cdn->nfad", rot, coh, rot)
+
+
+def select_chunk_polarized(topo, coh, ra, dec, r_avg, chunk: int,
+                           chunk_size: int, nsrc_alloc: int):
+    """Polarized branch of CoordinateRotation.select_chunk
+    (core/coords.py:99-150): cull and compact as :func:`select_chunk`, then
+    rotate the kept sources' coherency matrices into the alt/az basis.
+    Returns (coords (3, nsrc_alloc), coherency (nsrc_alloc, nfreq, 2, 2),
+    n, global indices kept)."""
+    topo = np.asarray(topo)
+    crd, _, n, index = select_chunk(topo, np.zeros(topo.shape[1]), chunk,
+                                    chunk_size, nsrc_alloc)
+    coh = np.asarray(coh)
+    out = np.zeros((nsrc_alloc,) + coh.shape[1:],
+                   np.complex128 if np.iscomplexobj(coh) else np.float64)
+    if n:
+        az, za = enu_to_az_za(crd[0, :n], crd[1, :n])
+        rot = coherency_rotation(np.asarray(ra)[index],
+                                 np.asarray(dec)[index], 0.5 * np.pi - za,
+                                 az, r_avg)
+        out[:n] = rotate_coherency(coh[index], rot)
+    return crd, out, n, index
 
 
 def prepare_grid(data_array, az, za, freq_array, freq: float,
