@@ -170,21 +170,17 @@ constexpr int kPRow = kPK + 1;  // LDS row length (one element of padding)
 // 256 threads = 4 waves in a 2x2 arrangement, each wave a 32x32 sub-tile made
 // of 2x2 blocks of 16x16.  One 16x16x4 MFMA consumes two complex columns:
 // lane l supplies k-slot g = l / 16 -> column g / 2, part g % 2 (re / im).
-template <typename T> struct PlainLds {
-  cplx<T> a[2][kPT * kPRow];
-  cplx<T> b[2][kPT * kPRow];
-};
-
 // One (slice, tile) item of a plain HERK: the slice's K tiles staged through
-// two LDS stages, the partial tile written to `part`.
+// two LDS stages (As for the tile's rows, Bs for its columns; separate
+// __shared__ arrays of the calling kernel, which the compiler aligns for
+// 16-byte LDS reads), the partial tile written to `part`.
 template <typename T>
 __device__ __forceinline__ void plain_item(
     const cplx<T> *__restrict__ z, int m, int64_t ldz, int64_t seg_stride,
     int64_t seg_len, int64_t kt_per_seg, int nb, int ntiles, int ktiles,
-    int ns, int slice, int tile, cplx<T> *__restrict__ part, PlainLds<T> &L) {
+    int ns, int slice, int tile, cplx<T> *__restrict__ part,
+    cplx<T> (&As)[2][kPT * kPRow], cplx<T> (&Bs)[2][kPT * kPRow]) {
   typedef typename PlainMfma<T>::acc acc_t;
-  cplx<T>(*As)[kPT * kPRow] = L.a;
-  cplx<T>(*Bs)[kPT * kPRow] = L.b;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int lrow = tid >> 2, lcol = (tid & 3) * 4;
@@ -297,7 +293,8 @@ __global__ __launch_bounds__(256) void herk_plain_kernel(
     int64_t seg_stride, int64_t seg_len_host,
     const int32_t *__restrict__ seg_len_dev, int nb, int ntiles,
     SliceCfg cfg, cplx<T> *__restrict__ part) {
-  __shared__ PlainLds<T> L;
+  __shared__ cplx<T> As[2][kPT * kPRow];
+  __shared__ cplx<T> Bs[2][kPT * kPRow];
   const int64_t seg_len = seg_columns(seg_len_host, seg_len_dev);
   const int64_t kt_per_seg = cdiv(seg_len, (int64_t)kPK);
   const int ktiles = (int)(kt_per_seg * nseg);
@@ -305,7 +302,7 @@ __global__ __launch_bounds__(256) void herk_plain_kernel(
   for_items((int64_t)ntiles * ns, [&](int64_t it) {
     plain_item<T>(z, m, ldz, seg_stride, seg_len, kt_per_seg, nb, ntiles,
                   ktiles, ns, (int)(it / ntiles), (int)(it % ntiles), part,
-                  L);
+                  As, Bs);
   });
 }
 
@@ -795,7 +792,8 @@ __global__ __launch_bounds__(SplitShape<TILE>::kThreads,
 template <typename T>
 __global__ __launch_bounds__(256) void herk_plain_grouped_kernel(
     const SplitProblem *__restrict__ probs, int nprob, GroupPlan plan) {
-  __shared__ PlainLds<T> L;
+  __shared__ cplx<T> As[2][kPT * kPRow];
+  __shared__ cplx<T> Bs[2][kPT * kPRow];
   const int64_t items = plan.item_off[nprob];
   for_items(items, [&](int64_t it) {
     const int p = find_segment(plan.item_off, nprob, it);
@@ -807,7 +805,7 @@ __global__ __launch_bounds__(256) void herk_plain_grouped_kernel(
     plain_item<T>((const cplx<T> *)P.t[0], P.m, P.ldz, P.seg_stride, seg_len,
                   kt_per_seg, P.nb, P.ntiles, (int)(kt_per_seg * P.nseg), ns,
                   (int)(local / P.ntiles), (int)(local % P.ntiles),
-                  (cplx<T> *)P.part, L);
+                  (cplx<T> *)P.part, As, Bs);
   });
 }
 
