@@ -78,6 +78,12 @@ def test_argument_errors_do_not_launch():
                            1, None, 2, 2, 8, None, 0, 1, 1.0, dummy, None,
                            None, 16, 8, None)
     assert rc == _lib.MV_ERR_ARG and "seg_stride" in _err()
+    rc = lib.mv_coherency_rotate(None, None, 10, None, None, 10, 1, None, 10,
+                                 None, None, 1, 1, 0, None, None)
+    assert rc == _lib.MV_ERR_ARG and "null pointer" in _err()
+    rc = lib.mv_coherency_rotate(dummy, dummy, 10, dummy, dummy, 5, 1, dummy,
+                                 10, dummy, dummy, 1, 1, 0, dummy, None)
+    assert rc == _lib.MV_ERR_ARG and "bad sizes" in _err()
 
 
 def test_call_raises_value_error():
