@@ -1,5 +1,5 @@
-"""Grouped split HERK (mv_herk_grouped) against single-problem launches, and
-its HIP-graph capture replayed several times."""
+"""Grouped HERK (mv_herk_grouped), split and plain forms, against
+single-problem launches, and its HIP-graph capture replayed several times."""
 
 import numpy as np
 import pytest
@@ -91,12 +91,45 @@ def test_grouped_matches_single_launches(tile, form, nterms, nprod):
                                    rtol=1e-13, atol=0)
 
 
-def test_grouped_graph_replays():
-    from matvis_amd.gpu.herk_group import GroupedBatch
-    print("grouped graph capture", flush=True)
-    form, nterms, nprod = FORM_SPLIT_BF16, 2, 3
-    probs = [_problem(*c, form, nterms, seed=10 + i) for i, c in enumerate(CASES)]
-    tiles = [64 if p["m"] <= 128 else 128 for p in probs]
+def _plain_problems(ct, seed):
+    """Plain-form problems over CASES: complex Z of dtype ct, zero beyond
+    each segment's column count."""
+    from matvis_amd.gpu.herk import Herk
+    from matvis_amd._lib import FORM_C64, FORM_C128
+    f = FORM_C128 if ct == np.complex128 else FORM_C64
+    probs = []
+    for i, (m, seg_len, nseg, n_dev) in enumerate(CASES):
+        rng = np.random.default_rng(seed + i)
+        seg = -(-seg_len // 16) * 16
+        z = (rng.normal(size=(m, nseg * seg))
+             + 1j * rng.normal(size=(m, nseg * seg))).astype(ct)
+        n = seg_len if n_dev is None else n_dev
+        for s_ in range(nseg):
+            z[:, s_ * seg + n:(s_ + 1) * seg] = 0
+        h = Herk(f, m)
+        probs.append(dict(terms=[D.to_dev(z)], m=m, ldz=nseg * seg,
+                          nseg=nseg, seg_stride=seg, seg_len=seg_len,
+                          seg_len_dev=None if n_dev is None else D.to_dev(
+                              np.array([n_dev], np.int32)),
+                          vacc=h.new_vacc(), ldv=h.ldv, scale=1.0,
+                          accumulate=False))
+    return f, probs
+
+
+@pytest.mark.parametrize("kind", ["split_bf16", "c128", "c64"])
+def test_grouped_graph_replays(kind):
+    from matvis_amd.gpu.herk_group import GroupedBatch, plain_tile
+    print(f"grouped graph capture {kind}", flush=True)
+    if kind == "split_bf16":
+        form, nterms, nprod = FORM_SPLIT_BF16, 2, 3
+        probs = [_problem(*c, form, nterms, seed=10 + i)
+                 for i, c in enumerate(CASES)]
+        tiles = [64 if p["m"] <= 128 else 128 for p in probs]
+    else:
+        nterms = nprod = 1
+        form, probs = _plain_problems(
+            np.complex128 if kind == "c128" else np.complex64, 50)
+        tiles = [plain_tile()] * len(probs)
     b = GroupedBatch(form, nterms, nprod, probs, tiles)
     b.run()
     torch.cuda.synchronize()
@@ -112,39 +145,24 @@ def test_grouped_graph_replays():
             assert torch.equal(p["vacc"], w), f"replay {rep} differs"
 
 
-
 @pytest.mark.parametrize("form", ["c128", "c64"])
 def test_grouped_plain_matches_single_launches(form):
     """Plain forms in one grouped launch (64-row tiles) against one plain
     launch per problem: FP64 to rounding, complex64 to FP32 accumulation."""
-    from matvis_amd._lib import FORM_C64, FORM_C128
     from matvis_amd.gpu.herk import Herk
     from matvis_amd.gpu.herk_group import HerkGroup, plain_tile
 
-    f = FORM_C128 if form == "c128" else FORM_C64
-    ct = np.complex128 if form == "c128" else np.complex64
     print(f"grouped plain {form}", flush=True)
-    probs, singles = [], []
-    for i, (m, seg_len, nseg, n_dev) in enumerate(CASES):
-        rng = np.random.default_rng(40 + i)
-        seg = -(-seg_len // 16) * 16
-        z = (rng.normal(size=(m, nseg * seg))
-             + 1j * rng.normal(size=(m, nseg * seg))).astype(ct)
-        n = seg_len if n_dev is None else n_dev
-        for s_ in range(nseg):
-            z[:, s_ * seg + n:(s_ + 1) * seg] = 0
-        zd = D.to_dev(z)
-        h = Herk(f, m)
-        probs.append(dict(terms=[zd], m=m, ldz=nseg * seg, nseg=nseg,
-                          seg_stride=seg, seg_len=seg_len,
-                          seg_len_dev=None if n_dev is None else D.to_dev(
-                              np.array([n_dev], np.int32)),
-                          vacc=h.new_vacc(), ldv=h.ldv, scale=1.0,
-                          accumulate=False))
+    f, probs = _plain_problems(
+        np.complex128 if form == "c128" else np.complex64, 40)
+    singles = []
+    for p in probs:
+        h = Herk(f, p["m"])
         v = h.new_vacc()
-        h.run([zd], ldz=nseg * seg, nseg=nseg, seg_stride=seg,
-              seg_len=seg_len, seg_len_dev=probs[-1]["seg_len_dev"], vacc=v,
-              accumulate=False, nslices=h.nslices(nseg * seg_len))
+        h.run(p["terms"], ldz=p["ldz"], nseg=p["nseg"],
+              seg_stride=p["seg_stride"], seg_len=p["seg_len"],
+              seg_len_dev=p["seg_len_dev"], vacc=v, accumulate=False,
+              nslices=h.nslices(p["nseg"] * p["seg_len"]))
         singles.append((h, v))
     g = HerkGroup(f, 1, 1, plain_tile(), probs)
     g.run()
