@@ -545,6 +545,27 @@ def serving_bench(seconds: float = 20.0, rate: float = 20.0,
     return out
 
 
+def host_output_rate(eng, ntimes: int, vis_per_time: int) -> dict:
+    """The drop-in API's rate: ``engine.run`` returning host arrays (each
+    time's output copied on a second stream into pinned buffers, then into
+    the caller's array), PCIe and host copies included; wall clock."""
+    import torch
+
+    out = np.empty((eng.nfreq, ntimes, eng.npairs, eng.nfeed, eng.nfeed),
+                   dtype=eng.ctype)
+    out.fill(0)  # first touch outside the timed region
+    eng.run(1, out[:, :1])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.run(ntimes, out)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    log(f"host output: {ntimes} times in {sec:.3f} s")
+    return dict(times=ntimes, seconds=sec,
+                vis_per_s=ntimes * vis_per_time / sec,
+                output_bytes=int(out.nbytes))
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -702,6 +723,9 @@ def main():
             / PEAK_TFLOPS[FORMS[name][0]],
             n_up_mean=float(np.mean(nups)),
         )
+        if name == forms[0]:
+            results[name]["host_output"] = host_output_rate(
+                eng, min(max(steps, 4), NTIMES_POOL), nfreq * vis_per_inst)
         # keep one instance (time 0, first frequencies) for the error figures
         out = eng.new_output()
         eng.compute_time(0, out)
