@@ -107,10 +107,14 @@ class CoordinateRotationERFA(CoordinateRotation):
 
     def rotate(self, t: int) -> None:
         """BCRS corrections when due (cpu/coords.py:186-215), then the
-        time's CIRS -> ENU matrix (cpu/coords.py:217-230)."""
+        time's CIRS -> ENU matrix (cpu/coords.py:217-230).  Due means more
+        than ``update_bcrs_every`` seconds from the last evaluation in
+        either direction: the reference compares the signed difference
+        (cpu/coords.py:193-197), so a step back in time would keep the later
+        time's corrections; for increasing times the two rules agree."""
         ctx = self.context(t)
         last = self._time_of_last_evaluation
-        if last is None or (self._jd[t] - self._jd[last]) * 86400.0 \
+        if last is None or abs(self._jd[t] - self._jd[last]) * 86400.0 \
                 > self.update_bcrs_every:
             call("mv_bcrs_update", D.ptr(self.eci), self.nsrc, self.nsrc,
                  bcrs_context_array(ctx).ctypes.data, 1e-6, D.ptr(self._bcrs),

@@ -432,3 +432,37 @@ def test_full_size_split_within_1e6_of_fp64(full_size):
         print(f"full size freq {fi}: split-BF16 vs FP64 normalised max "
               f"{e[0]:.2e} median {e[1]:.2e}", flush=True)
         assert e[0] < 1e-6
+
+
+def test_registered_rotations_are_repeatable():
+    """tests/test_coordrot.py:83-97: every registered rotation method gives
+    the same topocentric vectors when a time is rotated again, also after
+    other times (methods whose dependencies are absent are skipped)."""
+    from matvis_amd.core.coords import CoordinateRotation
+    from matvis_amd.gpu import coords as _  # noqa: F401
+    from matvis_amd import synth
+
+    print("rotation repeatability", flush=True)
+    ra, dec = synth.random_sky(500, 3)
+    times = synth.times_spaced(3, 30.0)
+    loc = synth.hera_location()
+    extra = {"CoordinateRotationMatrix": {
+        "rotations": astro.rotation_matrices(times, loc)}}
+    checked = 0
+    for name, cls in sorted(CoordinateRotation._methods.items()):
+        try:
+            co = cls(flux=np.ones(500), times=times, telescope_loc=loc,
+                     skycoords=astro.SkyCoord(ra, dec), precision=2,
+                     **extra.get(name, {}))
+        except ImportError:
+            continue
+        co.setup()
+        got = []
+        for t in (0, 1, 2, 0, 2):
+            co.rotate(t)
+            got.append(co.all_coords_topo.cpu().numpy().copy())
+        assert np.array_equal(got[0], got[3]), name
+        assert np.array_equal(got[2], got[4]), name
+        assert not np.array_equal(got[0], got[1]), name
+        checked += 1
+    assert checked >= 3
