@@ -194,3 +194,52 @@ def test_gpu_simulate_with_erfa_rotation():
     e = O.normalized_error(v[..., None, None], ref[..., None, None], nant, 1)
     print(f"  normalised max {e[0]:.2e}", flush=True)
     assert e[0] < 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_erfa_precision1_and_chunk_sizes():
+    """tests/test_coordrot.py:100-127 at precision 1 (within 150 arcsec of
+    the observed directions; single-precision output) and :147-168 (chunk
+    sizes 16, 24 and all sources give the same culled coordinates)."""
+    from matvis_amd import astro
+    from matvis_amd.gpu.coords import GPUCoordinateRotationERFA
+    import matvis_amd._device as D
+
+    print("erfa precision 1 and chunk sizes", flush=True)
+    lon, lat, hm = G["site"]
+    loc = astro.EarthLocation.from_geodetic(np.degrees(lon), np.degrees(lat),
+                                            hm)
+    arcsec = np.pi / (180 * 3600)
+
+    def make(chunk):
+        co = GPUCoordinateRotationERFA(
+            0.0, flux=np.ones(G["ra"].size), times=G["utc"],
+            telescope_loc=loc, skycoords=astro.SkyCoord(G["ra"], G["dec"]),
+            precision=1, chunk_size=chunk, source_buffer=1.0,
+            astrom=[ctx(i) for i in range(NT)])
+        co.setup()
+        return co
+
+    co = make(None)
+    for t in range(NT):
+        co.rotate(t)
+        got = D.to_host(co.all_coords_topo)
+        assert got.dtype == np.float32
+        got = got.astype(np.float64)
+        got /= np.linalg.norm(got, axis=0)
+        assert ang(got, G["obs_enu"][t]).max() < 150 * arcsec
+    small, mid = make(16), make(24)
+    for t in (0, NT - 1):
+        for c in (co, small, mid):
+            c.rotate(t)
+        whole, _, n_all = co.select_chunk(0, t)
+        whole = D.to_host(whole)[:, :n_all].copy()
+        parts = []
+        for c, size in ((small, 16), (mid, 24)):
+            cols = []
+            for k in range(-(-G["ra"].size // size)):
+                crd, _, n = c.select_chunk(k, t)
+                cols.append(D.to_host(crd)[:, :n].copy())
+            parts.append(np.concatenate(cols, axis=1))
+        for p in parts:
+            np.testing.assert_array_equal(p, whole)
