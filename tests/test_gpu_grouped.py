@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 # (m, columns per segment, segments, device column count or None)
 CASES = [(64, 1000, 1, None), (128, 3000, 2, 2500), (200, 500, 1, None),
          (350, 4000, 2, None), (700, 2000, 2, 1900), (1, 16, 1, None),
-         (65, 17, 1, 9), (130, 70, 2, None)]
+         (65, 17, 1, 9), (130, 70, 2, None), (90, 48, 2, 0)]
 
 
 def _problem(m, seg_len, nseg, n_dev, form, nterms, seed):
@@ -85,6 +85,8 @@ def test_grouped_matches_single_launches(tile, form, nterms, nprod):
         print(f"  case {c}: grouped vs single {e:.2e}", flush=True)
         # same terms, FP32 accumulation over different K-slices
         assert e < 1e-6, (c, e)
+        if c[3] == 0:  # no columns on the device: an exact zero
+            assert not got.any() and not want.any(), c
         # the diagonal is summed exactly in both: bit-identical up to the
         # order of the slice sums
         np.testing.assert_allclose(np.diag(got).real, np.diag(want).real,
@@ -172,3 +174,5 @@ def test_grouped_plain_matches_single_launches(form):
         e = _nerr(got, want)
         print(f"  case {c}: grouped vs single {e:.2e}", flush=True)
         assert e < (1e-13 if form == "c128" else 2e-6), (c, e)
+        if c[3] == 0:  # no columns on the device: an exact zero
+            assert not got.any() and not want.any(), c
