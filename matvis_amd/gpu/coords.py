@@ -116,8 +116,10 @@ class CoordinateRotationERFA(CoordinateRotation):
         last = self._time_of_last_evaluation
         if last is None or abs(self._jd[t] - self._jd[last]) * 86400.0 \
                 > self.update_bcrs_every:
+            # a named array: the library reads the 17 doubles during the call
+            ctx_arr = bcrs_context_array(ctx)
             call("mv_bcrs_update", D.ptr(self.eci), self.nsrc, self.nsrc,
-                 bcrs_context_array(ctx).ctypes.data, 1e-6, D.ptr(self._bcrs),
+                 ctx_arr.ctypes.data, 1e-6, D.ptr(self._bcrs),
                  self.nsrc, D.stream_ptr())
             self._time_of_last_evaluation = t
         self._rot = np.ascontiguousarray(atioq_matrix(ctx))

@@ -140,8 +140,11 @@ class GroupedBatch:
             g.run(stream)
 
     def capture(self):
-        """Record run() in a graph (after one eager run, so the kernels'
-        occupancy queries are cached outside the capture)."""
+        """Record run() in a graph.  The launches first run once eagerly, so
+        the kernels' occupancy queries are cached outside the capture: that
+        run is a real execution of the batch (problems with ``accumulate``
+        add into their accumulators), and a caller that captures in place of
+        a step must not also replay in that step."""
         self.run()
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()

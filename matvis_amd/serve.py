@@ -363,9 +363,13 @@ class VisibilityServer:
             if batch is None:
                 tiles = [tile_for(form, p["m"]) for p in probs]
                 batch = GroupedBatch(form, nterms, nprod, probs, tiles)
-                if self.graphs:
-                    batch.capture()
                 self._batches[key] = batch
+                if self.graphs:
+                    # capture() executes the launches once before recording
+                    # them: that execution is this step's, so no replay here
+                    # (a replay would add accumulating chunks twice)
+                    batch.capture()
+                    continue
             else:
                 batch.update(probs)
             if self.graphs:

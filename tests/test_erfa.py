@@ -121,9 +121,9 @@ def test_gpu_bcrs_update_matches_oracle(i):
     n = e.shape[1]
     src = D.to_dev(e, np.float64)
     out = D.zeros((3, n), np.float64)
-    call("mv_bcrs_update", D.ptr(src), n, n,
-         bcrs_context_array(ctx(i)).ctypes.data, 1e-6, D.ptr(out), n,
-         D.stream_ptr())
+    ctx_arr = bcrs_context_array(ctx(i))  # alive for the whole call
+    call("mv_bcrs_update", D.ptr(src), n, n, ctx_arr.ctypes.data, 1e-6,
+         D.ptr(out), n, D.stream_ptr())
     np.testing.assert_allclose(D.to_host(out), O.erfa_bcrs(e, ctx(i)),
                                rtol=0, atol=4e-16)
 

@@ -176,3 +176,90 @@ def test_grouped_plain_matches_single_launches(form):
         assert e < (1e-13 if form == "c128" else 2e-6), (c, e)
         if c[3] == 0:  # no columns on the device: an exact zero
             assert not got.any() and not want.any(), c
+
+
+def _numpy_upper_nerr(got, z, m):
+    """Normalised max error of the upper triangle of `got` against numpy's
+    FP64 conj(Z) Z^T of the complex Z."""
+    zh = D.to_host(z).astype(np.complex128)
+    ref = np.conj(zh) @ zh.T
+    d = np.sqrt(np.abs(np.diag(ref)))
+    d[d == 0] = 1
+    iu = np.triu_indices(m)
+    return float((np.abs(got[:m, :m] - ref) / (d[:, None] * d[None]))[iu].max())
+
+
+@pytest.mark.parametrize("kind", ["split_bf16", "c128"])
+def test_grouped_accumulate_eager_and_replayed(kind):
+    """accumulate=True adds into a pre-filled accumulator: the eager grouped
+    launch, the graph capture's own run and every replay (the accumulator
+    reset to the pre-filled values between them) give the pre-filled values
+    plus the accumulate=False result, which matches numpy FP64."""
+    from matvis_amd.gpu.herk_group import GroupedBatch, HerkGroup, plain_tile
+    print(f"grouped accumulate {kind}", flush=True)
+    if kind == "split_bf16":
+        form, nterms, nprod = FORM_SPLIT_BF16, 2, 3
+        probs = [_problem(*c, form, nterms, seed=80 + i)
+                 for i, c in enumerate(CASES)]
+        zs = [p["z"] for p in probs]
+        tiles = [64 if p["m"] <= 128 else 128 for p in probs]
+        # against numpy on the unsplit Z: the two-term bf16 representation
+        # (2^-17 relative per element) is not averaged down over the few
+        # hundred columns of these problems (the bound of the split-HERK
+        # kernel tests)
+        bound = 3e-5
+    else:
+        nterms = nprod = 1
+        form, probs = _plain_problems(np.complex128, 90)
+        zs = [p["terms"][0] for p in probs]
+        tiles = [plain_tile()] * len(probs)
+        bound = 1e-12
+    # the accumulate=False result, per tile class as GroupedBatch runs it
+    GroupedBatch(form, nterms, nprod, probs, tiles).run()
+    torch.cuda.synchronize()
+    base = [p["vacc"].clone() for p in probs]
+    for p, b, z, c in zip(probs, base, zs, CASES):
+        m, n = p["m"], (c[1] if c[3] is None else c[3])
+        # numpy sees only the columns the device counted
+        zc = z.view(m, p["nseg"], p["seg_stride"])[:, :, :n].reshape(m, -1)
+        e = _numpy_upper_nerr(D.to_host(b), zc, m)
+        assert e < bound, (c, e)
+    rng = np.random.default_rng(5)
+    pre = [D.to_dev(rng.normal(size=b.shape) + 1j * rng.normal(size=b.shape))
+           for b in base]
+
+    def reset():
+        for p, q in zip(probs, pre):
+            p["vacc"].copy_(q)
+        torch.cuda.synchronize()
+
+    def check(what):
+        torch.cuda.synchronize()
+        for p, b, q, c in zip(probs, base, pre, CASES):
+            m = p["m"]
+            got = D.to_host(p["vacc"])[:m, :m]
+            want = (D.to_host(q) + D.to_host(b))[:m, :m]
+            iu = np.triu_indices(m)
+            # one rounding of the add (fused or not) per element
+            np.testing.assert_allclose(got[iu], want[iu], rtol=1e-15,
+                                       atol=1e-15 * np.abs(want).max(),
+                                       err_msg=f"{what} {c}")
+
+    for p in probs:
+        p["accumulate"] = True
+    reset()
+    GroupedBatch(form, nterms, nprod, probs, tiles).run()
+    check("eager")
+    reset()
+    batch = GroupedBatch(form, nterms, nprod, probs, tiles)
+    batch.capture()  # one eager execution, then the recording
+    check("capture")
+    for rep in range(3):
+        reset()
+        batch.replay()
+        check(f"replay {rep}")
+    # a single HerkGroup launch of the same problems accumulates the same way
+    if kind == "c128":
+        reset()
+        HerkGroup(form, 1, 1, plain_tile(), probs).run()
+        check("group")

@@ -165,3 +165,31 @@ def test_shortest_first_policy():
     assert order != [0, 1, 2]  # not the arrival order
     for r, a in zip(reqs, alone):
         assert r.error is None and np.array_equal(r.result, a)
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_multichunk_request_matches_simulate_multi(graphs):
+    """A request cut into source chunks (min_chunks=3): the chunks after the
+    first accumulate into the same FP64 accumulator inside the grouped
+    launches, captured or not, alone and beside other requests; the result
+    equals simulate_multi's."""
+    import oracle as O
+    print(f"serve multichunk graphs={graphs}", flush=True)
+    specs = [(20, 2400, 2, 2, True, "GPUMatMul", 31),
+             (20, 2400, 2, 2, True, "GPUMatMulSplitBF16", 32),
+             (12, 900, 1, 2, False, "GPUMatMul", 33)]
+    kws = [dict(_kwargs(*s), min_chunks=3) for s in specs]
+    srv = VisibilityServer(graphs=graphs)
+    reqs = [srv.submit(SimRequest(k)) for k in kws]
+    srv.run_until_idle(timeout=300)
+    for s, k, r in zip(specs, kws, reqs):
+        assert r.error is None, r.error
+        want = simulate_multi(**k)
+        nf = 2 if s[4] else 1
+        bound = 1e-12 if s[5] == "GPUMatMul" else 1e-6
+        for fi in range(want.shape[0]):
+            a, b = r.result[fi], want[fi]
+            if not s[4]:
+                a, b = a[..., None, None], b[..., None, None]
+            e = O.normalized_error(a, b, s[0], nf)[0]
+            assert e < bound, (s, fi, e)
