@@ -121,19 +121,49 @@ def is_analytic_beam(beam) -> bool:
     return hasattr(inner, "get_sigma")
 
 
+#: Distance (Hz) within which a requested frequency takes the nearest beam
+#: node's data instead of an interpolation (UVBeam.interp's freq_interp_tol
+#: default, reached from core/beams.py:82-93).
+FREQ_INTERP_TOL = 1.0
+
+
 def _interp_freq(data: np.ndarray, freqs: np.ndarray, freq: float):
-    """Linear interpolation of data[..., f, :, :] to one frequency."""
+    """data[:, :, f] at one frequency, as ``UVBeam.interp(freq_array=[freq])``
+    with its defaults (core/beams.py:82-93):
+
+    * within ``FREQ_INTERP_TOL`` of a node: that node's data;
+    * otherwise a cubic spline through all nodes along the frequency axis
+      (``freq_interp_kind="cubic"``: scipy's ``interp1d``, not-a-knot ends,
+      real and imaginary parts separately), which needs at least 4 nodes and
+      the frequency inside their range; both conditions raise ValueError as
+      there.
+
+    A beam with a single frequency is used unchanged at every frequency (a
+    frequency-independent grid beam); the reference raises for it instead
+    ("Only one frequency in UVBeam so cannot interpolate")."""
     if data.shape[2] == 1:
         return data[:, :, 0]
-    order = np.argsort(freqs)
-    freqs = freqs[order]
-    data = data[:, :, order]
-    if not freqs[0] <= freq <= freqs[-1]:
+    freqs = np.asarray(freqs, dtype=np.float64)
+    dist = np.abs(freqs - freq)
+    near = int(np.argmin(dist))
+    if dist[near] < FREQ_INTERP_TOL:
+        return data[:, :, near]
+    if not freqs.min() <= freq <= freqs.max():
         raise ValueError(f"frequency {freq} Hz outside the beam's range "
-                         f"[{freqs[0]}, {freqs[-1]}]")
-    k = int(np.clip(np.searchsorted(freqs, freq) - 1, 0, freqs.size - 2))
-    w = (freq - freqs[k]) / (freqs[k + 1] - freqs[k])
-    return (1.0 - w) * data[:, :, k] + w * data[:, :, k + 1]
+                         f"[{freqs.min()}, {freqs.max()}]")
+    if freqs.size < 4:
+        raise ValueError(
+            f"frequency {freq} Hz is between beam frequencies and the cubic "
+            f"frequency interpolation needs at least 4 of them "
+            f"(the beam has {freqs.size})")
+    from scipy.interpolate import interp1d
+
+    def spline(x):
+        return interp1d(freqs, x, kind="cubic", axis=2)(freq)
+
+    if np.iscomplexobj(data):
+        return spline(data.real) + 1j * spline(data.imag)
+    return spline(data)
 
 
 def grid_planes(beam, freq: float, polarized: bool, beam_type=None):
@@ -143,12 +173,11 @@ def grid_planes(beam, freq: float, polarized: bool, beam_type=None):
     ``(nax, nfeed, nza, naz')`` -- the ``(nbeam, nax, nfeed, nza, naz)`` layout
     of gpu/beams.py:143-145 for one beam.
 
-    * frequency: the single plane, or linear interpolation between the two
-      bracketing frequencies (pyuvdata's own frequency interpolation is not
-      restated; unpinned);
     * unpolarized: an E-field beam becomes the xx power beam
       ``|E[0, x]|^2 + |E[1, x]|^2`` (core/beams.py:15-32); a power beam keeps
       its first polarization;
+    * frequency, after the power conversion as there (core/beams.py:65-66
+      before :82-93): :func:`_interp_freq`;
     * azimuth wrap, as pyuvdata's ``UVBeam._prepare_coordinate_data`` that
       gpu/beams.py:15-19 calls: when span + spacing is within one spacing of
       2 pi, 3 columns of wrapped data are added on each side and ``azmin``
@@ -166,23 +195,26 @@ def grid_planes(beam, freq: float, polarized: bool, beam_type=None):
         raise ValueError("a grid beam needs at least 2 points on each axis")
     if abs(za[0]) > 1e-12:
         raise ValueError("the zenith-angle grid must start at 0")
-    data = _interp_freq(np.asarray(inner.data_array),
-                        np.asarray(inner.freq_array, float), freq)
-    # data: (Naxes_vec, Nfeeds|Npols, nza, naz); its meaning follows the
-    # stored beam's own type, the requested use follows `bt`
+    data = np.asarray(inner.data_array)
+    freqs = np.asarray(inner.freq_array, float)
+    # data: (Naxes_vec, Nfeeds|Npols, Nfreqs, nza, naz); its meaning follows
+    # the stored beam's own type, the requested use follows `bt`
     stored = getattr(inner, "beam_type", "efield")
     if polarized:
         if bt != "efield" or stored != "efield":
             raise ValueError("beam type must be efield if using polarized=True")
-        planes = data[:2, :2].astype(np.complex128)
+        planes = _interp_freq(data[:2, :2], freqs, freq).astype(np.complex128)
         is_power = False
     else:
         if stored == "efield":
-            e = data[:, 0]  # feed x, every vector axis
-            power = (np.abs(e) ** 2).sum(axis=0)
+            # feed x, every vector axis: (nvec, Nfreqs, ...); the power in
+            # FP64 whatever the stored precision
+            e = data[:, 0].astype(np.complex128)
+            power = (e.real ** 2 + e.imag ** 2).sum(axis=0)
         else:
             power = np.real(data[0, 0])
-        planes = power[None, None].astype(np.float64)
+        planes = _interp_freq(power[None, None], freqs, freq).astype(
+            np.float64)
         is_power = True
     step = az[1] - az[0]
     span = abs(az[0] - az[-1]) + step

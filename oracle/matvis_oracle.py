@@ -29,11 +29,19 @@ Parity status per function (see ``tests/test_oracle.py``):
 ``rotate``                  pinned: brute force vs an explicit matrix product
 ``select_chunk``            pinned: ascending-order compaction, zero padding
 ``enu_to_az_za``            pinned: E/N/U conventions (test_coordinates.py:108-160)
-``prepare_grid``            **parity unpinned** for its two pyuvdata steps
-                            (frequency interpolation, the 3-column azimuth
-                            padding of _prepare_coordinate_data); pinned by
-                            invariants only: grid nodes are reproduced, padded
-                            columns equal the wrapped data
+``prepare_grid``            pinned: continuity across az = 0 / 2 pi and
+                            accuracy on both sides of the seam for a
+                            trigonometric-polynomial beam (a missing wrap or
+                            a wrong azmin fails); power conversion before
+                            the frequency step; **parity unpinned**: the
+                            3-column padding width of pyuvdata's
+                            _prepare_coordinate_data
+``beam_at_frequency``       pinned: scipy interp1d(kind="cubic"), the
+                            routine UVBeam.interp calls; beams cubic in
+                            frequency reproduced exactly off the nodes (a
+                            linear restatement fails); node within 1 Hz;
+                            **parity unpinned**: that "cubic" is pyuvdata's
+                            default kind (its documentation; absent here)
 ``interp_grid_beam``        pinned: scipy map_coordinates; node identity and
                             linear exactness (test_beam_interp_gpu.py:12-64)
 ``gaussian_beam``           **parity unpinned**: restates pyuvdata's
@@ -263,17 +271,50 @@ def select_chunk_polarized(topo, coh, ra, dec, r_avg, chunk: int,
     return crd, out, n, index
 
 
+#: UVBeam.interp's freq_interp_tol default (Hz).
+FREQ_INTERP_TOL = 1.0
+
+
+def beam_at_frequency(d, freq_array, freq: float):
+    """``d[:, :, f]`` at one frequency, as ``UVBeam.interp(freq_array=[freq])``
+    with its defaults, reached from core/beams.py:82-93 (pyuvdata 3.2.4,
+    ``uv.lock:2932-2968``; absent here): the nearest node's data when it is
+    within ``freq_interp_tol`` = 1 Hz, otherwise ``freq_interp_kind="cubic"``,
+    i.e. scipy's ``interp1d(kind="cubic")`` along the frequency axis, real
+    and imaginary parts separately (at least 4 nodes; frequencies outside
+    the nodes' range raise).  A single-frequency beam is returned unchanged
+    (the reference raises there; here it is a frequency-independent beam).
+    """
+    from scipy.interpolate import interp1d
+
+    fa = np.atleast_1d(np.asarray(freq_array, dtype=np.float64))
+    if fa.size == 1:
+        return d[:, :, 0]
+    dist = np.abs(fa - freq)
+    if dist.min() < FREQ_INTERP_TOL:
+        return d[:, :, int(np.argmin(dist))]
+    if not fa.min() <= freq <= fa.max():
+        raise ValueError("frequency outside the beam's range")
+    if fa.size < 4:
+        raise ValueError("cubic frequency interpolation needs 4 frequencies")
+    out = interp1d(fa, np.real(d), kind="cubic", axis=2)(freq)
+    if np.iscomplexobj(d):
+        out = out + 1j * interp1d(fa, np.imag(d), kind="cubic", axis=2)(freq)
+    return out
+
+
 def prepare_grid(data_array, az, za, freq_array, freq: float,
                  polarized: bool, beam_type: str = "efield"):
     """Interpolation grid of one UVBeam-shaped az/za beam at one frequency.
 
     The preparation the reference applies before ``gpu_beam_interpolation``:
 
+    * unpolarized: ``prepare_beam_unpolarized`` (core/beams.py:15-32), run
+      first (core/beams.py:65-66): an E-field beam becomes the xx power beam
+      ``sum_vec |E[vec, x]|^2`` (the first feed) at every frequency node, a
+      power beam keeps its first polarization;
     * frequency: ``UVBeam.interp(freq_array=[freq])`` (core/beams.py:82-93),
-      restated as linear interpolation between the bracketing frequencies;
-    * unpolarized: ``prepare_beam_unpolarized`` (core/beams.py:15-32): an
-      E-field beam becomes the xx power beam ``sum_vec |E[vec, x]|^2`` (the
-      first feed), a power beam keeps its first polarization;
+      :func:`beam_at_frequency`;
     * ``prepare_for_map_coords`` (gpu/beams.py:15-19), which takes pyuvdata's
       ``UVBeam._prepare_coordinate_data``: when the azimuth axis covers the
       circle (span + spacing within one spacing of 2 pi) the grid is padded
@@ -281,26 +322,24 @@ def prepare_grid(data_array, az, za, freq_array, freq: float,
       spacing, ``azmin`` the padded axis's minimum.
 
     Returns ``(planes (nax, nfeed, nza, naz'), daz, dza, azmin)``.
-    PARITY UNPINNED for the frequency interpolation and the padding rule
-    (pyuvdata 3.2.4 algorithms, absent here); pinned by invariants only
-    (nodes reproduce the grid; a periodic beam agrees at az and az + 2 pi).
+    Pinned: the wrap and ``azmin`` by continuity across az = 0 / 2 pi and
+    by accuracy against a trigonometric-polynomial beam on both sides of the
+    seam; the frequency step by exact reproduction of beams cubic in
+    frequency.  PARITY UNPINNED: the padding width of 3 columns (it sets the
+    cubic prefilter's boundary near the seam) and pyuvdata's default kind
+    "cubic" itself (from its documentation; pyuvdata is absent here).
     """
     data = np.asarray(data_array)
     az = np.asarray(az, dtype=np.float64)
     za = np.asarray(za, dtype=np.float64)
-    fa = np.atleast_1d(np.asarray(freq_array, dtype=np.float64))
-    if fa.size == 1:
-        d = data[:, :, 0]
-    else:
-        i = int(np.clip(np.searchsorted(fa, freq) - 1, 0, fa.size - 2))
-        w = (freq - fa[i]) / (fa[i + 1] - fa[i])
-        d = (1 - w) * data[:, :, i] + w * data[:, :, i + 1]
     if polarized:
-        planes = d[:2, :2].astype(np.complex128)
+        d = data[:2, :2].astype(np.complex128)
     elif beam_type == "efield":
-        planes = (np.abs(d[:, 0]) ** 2).sum(axis=0)[None, None]
+        e = data[:, 0].astype(np.complex128)
+        d = (e.real ** 2 + e.imag ** 2).sum(axis=0)[None, None]
     else:
-        planes = np.real(d[0, 0])[None, None].astype(np.float64)
+        d = np.real(data[0, 0])[None, None].astype(np.float64)
+    planes = beam_at_frequency(d, freq_array, freq)
     step = az[1] - az[0]
     span = abs(az[0] - az[-1]) + step
     if np.isclose(span, 2 * np.pi, atol=step):

@@ -68,6 +68,40 @@ def test_grid_beam_fp64_matches_oracle(order):
     assert e[0] < 1e-12
 
 
+@pytest.mark.parametrize("polarized", [True, False])
+def test_grid_beam_off_node_frequency_matches_oracle(polarized):
+    """A grid beam with 5 frequency nodes whose patterns change with
+    frequency (a cubic in frequency of three harmonic patterns), simulated
+    at 137.3 MHz, between nodes: the cubic frequency step (after the power
+    conversion when unpolarized) on the host, then the device pipeline,
+    against the oracle at FP64."""
+    from matvis_amd import synth
+    from matvis_amd.beams import GridBeam
+
+    p = small_problem(nant=10, nsrc=1200, ntimes=2, polarized=polarized,
+                      beam="grid", order=3)
+    pats = [synth.harmonic_efield_beam(seed=20 + k, naz=72, nza=19,
+                                       dtype=np.complex128).data_array[:, :, 0]
+            for k in range(3)]
+    nodes = np.array([100e6, 125e6, 150e6, 175e6, 200e6])
+    t = (nodes - 150e6) / 50e6
+    data = np.stack([pats[0] + ti * pats[1] + ti ** 3 * pats[2] for ti in t],
+                    axis=2)
+    b0 = p["beam_list"][0]
+    beam = GridBeam(data, b0.axis1_array, b0.axis2_array, freq_array=nodes)
+    freq = 137.3e6
+    planes, daz, dza, azmin = O.prepare_grid(
+        data, beam.axis1_array, beam.axis2_array, nodes, freq, polarized)
+    obeam = {"kind": "grid", "planes": planes[None], "daz": [daz],
+             "dza": [dza], "azmin": [azmin], "order": 3}
+    v = run_gpu(p, beam_list=[beam], freq=freq)
+    ref = run_oracle(p, beam=obeam, freq=freq)
+    e = nerr(v, ref, 10, polarized)
+    print(f"off-node frequency pol={polarized}: normalised max {e[0]:.2e}",
+          flush=True)
+    assert e[0] < 1e-12
+
+
 def test_multichunk_source_buffer_and_antpairs():
     p = small_problem(nant=10, nsrc=3001, ntimes=2, polarized=True)
     pairs = np.array([(0, 0), (3, 7), (7, 3), (9, 1), (2, 2)])

@@ -64,11 +64,11 @@ def test_grid_planes_matches_oracle_preparation(polarized, naz, full):
         az = np.radians(np.arange(naz) * 2.005586592178771)
     za = np.radians(np.linspace(0, 90, 10))
     rng = np.random.default_rng(naz)
-    shape = (2, 2, 3, za.size, az.size)
+    freqs = np.array([100e6, 130e6, 150e6, 180e6, 200e6])
+    shape = (2, 2, freqs.size, za.size, az.size)
     data = rng.normal(size=shape) + 1j * rng.normal(size=shape)
-    freqs = np.array([100e6, 150e6, 200e6])
     b = GridBeam(data, az, za, freq_array=freqs)
-    for f in (100e6, 137.5e6, 200e6):
+    for f in (100e6, 137.5e6, 150e6 + 0.5, 200e6):
         planes, daz, dza, azmin, _ = grid_planes(b, f, polarized)
         want, wdaz, wdza, wazmin = O.prepare_grid(data, az, za, freqs, f,
                                                   polarized)
@@ -94,15 +94,35 @@ def test_grid_planes_rejects_bad_grids():
 
 
 def test_grid_planes_frequency_interpolation():
+    """Off the nodes, the cubic frequency step reproduces a beam cubic in
+    frequency; a node within 1 Hz gives its data; outside the range, or
+    between fewer than 4 nodes, it raises; one node is used at every
+    frequency."""
     az = np.radians(np.arange(8) * 45.0)
     za = np.radians([0.0, 45.0, 90.0])
-    data = np.ones((2, 2, 2, 3, 8), complex)
-    data[:, :, 1] *= 3.0
-    b = GridBeam(data, az, za, freq_array=[100e6, 200e6])
-    planes, *_ = grid_planes(b, 150e6, True)
-    np.testing.assert_allclose(planes, 2.0)
+    nodes = np.array([100e6, 125e6, 150e6, 175e6, 200e6])
+    rng = np.random.default_rng(2)
+    coef = rng.normal(size=(4, 2, 2, 3, 8))
+
+    def at(nu):
+        t = (nu - 150e6) / 50e6
+        return coef[0] + coef[1] * t + coef[2] * t ** 2 + coef[3] * t ** 3
+
+    data = np.stack([at(nu) for nu in nodes], axis=2).astype(complex)
+    b = GridBeam(data, az, za, freq_array=nodes)
+    planes, *_ = grid_planes(b, 140e6, True)
+    np.testing.assert_allclose(planes[..., 3:-3], at(140e6), rtol=1e-12,
+                               atol=1e-12)
+    planes, *_ = grid_planes(b, 125e6 + 0.3, True)
+    np.testing.assert_array_equal(planes[..., 3:-3], data[:, :, 1])
     with pytest.raises(ValueError, match="outside the beam's range"):
         grid_planes(b, 250e6, True)
+    b3 = GridBeam(data[:, :, :3], az, za, freq_array=nodes[:3])
+    with pytest.raises(ValueError, match="at least 4"):
+        grid_planes(b3, 110e6, True)
+    b1 = GridBeam(data[:, :, :1], az, za, freq_array=nodes[:1])
+    planes, *_ = grid_planes(b1, 190e6, True)
+    np.testing.assert_array_equal(planes[..., 3:-3], data[:, :, 0])
 
 
 def test_wrangle_beams_errors():

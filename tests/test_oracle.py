@@ -303,3 +303,120 @@ def test_normalized_error_detects_a_planted_mistake(rng):
     # the transposed feed layout is a real mistake the metric must flag
     e2 = O.normalized_error(v.transpose(0, 1, 3, 2), v, 4, 2)
     assert e2[0] > 1e-3
+
+
+# ---------------------------------------------------------------- beam grid
+# preparation: azimuth seam and frequency step
+
+def _trig_beam(naz=72, nza=19):
+    """A beam that is an exact trigonometric polynomial in azimuth, smooth
+    in zenith angle, on a grid covering the circle (5-degree spacing)."""
+    az = np.radians(np.arange(naz) * 360.0 / naz)
+    za = np.radians(np.linspace(0, 90, nza))
+
+    def f(a, z):
+        return ((1 + 0.5 * np.cos(a) + 0.3 * np.sin(2 * a)
+                 + 0.2 * np.cos(3 * a)) * (1 + 0.3 * np.cos(2 * z)))
+
+    AZ, ZA = np.meshgrid(az, za)
+    data = np.broadcast_to(f(AZ, ZA).astype(complex), (2, 2, 1, za.size,
+                                                       az.size)).copy()
+    # bilinear error bound h^2/8 (max|f_az az| + max|f_za za|), h = 5 deg
+    h = np.radians(5.0)
+    bound = h * h / 8 * ((0.5 + 0.3 * 4 + 0.2 * 9) * 1.3 + 1.2 * 2.0)
+    # max |df/daz|
+    slope = (0.5 + 0.6 + 0.6) * 1.3
+    return az, za, data, f, bound, slope
+
+
+def _seam_errors(planes, daz, dza, azmin, f, order):
+    rng = np.random.default_rng(3)
+    h = np.radians(5.0)
+    pa = np.mod(2 * np.pi - 4 * h + np.linspace(0, 8 * h, 401), 2 * np.pi)
+    pz = rng.uniform(np.radians(20), np.radians(70), pa.size)
+    s = O.interp_grid_beam(planes[None], [daz], [dza], [azmin], pa, pz,
+                           order)[0, 0, 0].real
+    eps = 1e-9
+    c = O.interp_grid_beam(planes[None], [daz], [dza], [azmin],
+                           np.array([2 * np.pi - eps, eps]),
+                           np.array([0.7, 0.7]), order)[0, 0, 0].real
+    return np.abs(s - f(pa, pz)).max(), abs(c[0] - c[1]), eps
+
+
+@pytest.mark.parametrize("order", [1, 3])
+def test_prepare_grid_azimuth_seam(order):
+    """Interpolation across az = 0 / 2 pi: continuous at the seam (within
+    the function's own change over 2e-9 rad) and within the bilinear error
+    bound of the true beam on both sides (the bicubic spline is inside it
+    too).  A grid without the wrap, or with azmin left at the first
+    azimuth, fails both checks by orders of magnitude."""
+    az, za, data, f, bound, slope = _trig_beam()
+    planes, daz, dza, azmin = O.prepare_grid(data, az, za, [150e6], 150e6,
+                                             True)
+    err, jump, eps = _seam_errors(planes, daz, dza, azmin, f, order)
+    assert err < bound, err
+    assert jump < 2 * eps * slope + 1e-12, jump
+    # planted: no wrap (the unpadded grid at its own azimuth origin)
+    e2, j2, _ = _seam_errors(planes[..., 3:-3], daz, dza, az.min(), f, order)
+    assert e2 > 100 * bound and j2 > 0.1, (e2, j2)
+    # planted: padded grid, azmin not moved below the first azimuth
+    e3, _, _ = _seam_errors(planes, daz, dza, az.min(), f, order)
+    assert e3 > 10 * bound, e3
+
+
+def _freq_cubic_beam(nodes_mhz):
+    """E-field data cubic in frequency: A + B t + C t^2 + D t^3 with
+    t = (nu - 150 MHz) / 50 MHz, on a small grid."""
+    rng = np.random.default_rng(11)
+    shape = (2, 2, 5, 8)
+    coef = [rng.normal(size=shape) + 1j * rng.normal(size=shape)
+            for _ in range(4)]
+
+    def at(nu):
+        t = (nu - 150e6) / 50e6
+        return coef[0] + coef[1] * t + coef[2] * t ** 2 + coef[3] * t ** 3
+
+    nodes = np.asarray(nodes_mhz, float) * 1e6
+    data = np.stack([at(nu) for nu in nodes], axis=2)
+    return data, nodes, at
+
+
+def test_beam_at_frequency_reproduces_cubics():
+    """Off the nodes, a beam cubic in frequency comes back exactly (a
+    not-a-knot cubic spline reproduces cubics); linear interpolation
+    between the bracketing nodes would miss it by ~1e-2.  Within 1 Hz of a
+    node: that node's data."""
+    data, nodes, at = _freq_cubic_beam([100, 120, 140, 160, 180, 200])
+    for nu in (103.7e6, 133.3e6, 151.0e6, 199.2e6):
+        got = O.beam_at_frequency(data, nodes, nu)
+        np.testing.assert_allclose(got, at(nu), rtol=1e-12,
+                                   atol=1e-12 * np.abs(at(nu)).max())
+        k = np.searchsorted(nodes, nu) - 1
+        w = (nu - nodes[k]) / (nodes[k + 1] - nodes[k])
+        lin = (1 - w) * data[:, :, k] + w * data[:, :, k + 1]
+        assert np.abs(lin - at(nu)).max() > 1e-3
+    np.testing.assert_array_equal(O.beam_at_frequency(data, nodes,
+                                                      140e6 + 0.4),
+                                  data[:, :, 2])
+    with pytest.raises(ValueError):
+        O.beam_at_frequency(data, nodes, 99e6)
+    with pytest.raises(ValueError):
+        O.beam_at_frequency(data[:, :, :3], nodes[:3], 110e6)
+
+
+def test_prepare_grid_power_before_frequency_step():
+    """Unpolarized: the E-field becomes power at every node first, then the
+    frequency step (core/beams.py:65-66 before :82-93).  For an E-field
+    cubic in frequency the two orders differ; the planes follow the
+    reference's."""
+    from scipy.interpolate import interp1d
+    data, nodes, at = _freq_cubic_beam([100, 120, 140, 160, 180, 200])
+    az = np.radians(np.arange(8) * 5.0)  # partial circle: no padding
+    za = np.radians(np.linspace(0, 40, 5))
+    nu = 133.3e6
+    planes, *_ = O.prepare_grid(data, az, za, nodes, nu, False)
+    power_nodes = (np.abs(data[:, 0]) ** 2).sum(0)  # (nfreq, nza, naz)
+    want = interp1d(nodes, power_nodes, kind="cubic", axis=0)(nu)
+    np.testing.assert_allclose(planes[0, 0], want, rtol=1e-13)
+    other = (np.abs(at(nu)[:, 0]) ** 2).sum(0)
+    assert np.abs(other - want).max() > 1e-6 * np.abs(want).max()
