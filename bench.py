@@ -379,9 +379,6 @@ def diffuse_bench(ntimes: int = 2, warmup: int = 1):
     return out
 
 
-SERVE_NANTS = (32, 128, 350, 512)
-
-
 def serving_requests(seconds: float, rate: float, seed: int = 11):
     """A Poisson request stream: arrival offsets and request keywords.
 
@@ -390,39 +387,23 @@ def serving_requests(seconds: float, rate: float, seed: int = 11):
     deg, power-law S^-2 fluxes on [0.1, 10] Jy); nfreq uniform on [1, 64]
     over 100-200 MHz; ntime uniform on [1, 16], 2 min apart; polarized with
     p = 0.5; plain FP64 or split-BF16 with p = 0.5; one shared harmonic
-    E-field beam (bicubic)."""
-    from matvis_amd import astro, synth
+    E-field beam (bicubic).  Sizes from ``synth.serving_draw``, keywords
+    from ``synth.ServingSky.request``."""
+    from matvis_amd import synth
 
     rng = np.random.default_rng(seed)
     n = max(1, int(rate * seconds * 2) + 16)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
     arrivals = arrivals[arrivals < seconds]
-    loc = synth.hera_location()
-    beam = synth.harmonic_efield_beam(seed=4, naz=360, nza=91)
-    times = synth.times_spaced(16, 2.0)
-    rots = astro.rotation_matrices(times, loc)
-    ants = {na: synth.hex_array(na, 14.6, 0.02, seed=na) for na in SERVE_NANTS}
+    sky = synth.ServingSky()
     reqs = []
     for i in range(arrivals.size):
-        nant = int(rng.choice(SERVE_NANTS))
-        nsrc = int(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
-        nfreq = int(rng.integers(1, 65))
-        ntime = int(rng.integers(1, 17))
-        pol = bool(rng.random() < 0.5)
-        fp64 = bool(rng.random() < 0.5)
-        ra, dec = synth.random_sky(nsrc, seed * 1000 + i, -90.0, 30.0)
-        flux = synth.powerlaw_fluxes(nsrc, seed * 1000 + i + 1)
-        freqs = np.linspace(100e6, 200e6, nfreq)
-        kw = dict(antpos=ants[nant], freqs=freqs, times=times[:ntime],
-                  skycoords=astro.SkyCoord(ra, dec), telescope_loc=loc,
-                  fluxes=np.repeat(flux[:, None], nfreq, 1),
-                  beam_list=[beam], polarized=pol, precision=2,
-                  coord_method="CoordinateRotationMatrix",
-                  coord_method_params={"rotations": rots[:ntime]},
-                  beam_spline_opts={"order": 3},
-                  matprod_method="GPUMatMul" if fp64 else "GPUMatMulSplitBF16")
-        label = ("fp64" if fp64 else "bf16") + ("/pol" if pol else "/unpol")
-        work = ntime * nfreq * (nant * (2 if pol else 1)) ** 2 * nsrc
+        d = synth.serving_draw(rng)
+        kw = sky.request(seed=seed * 1000 + i, **d)
+        label = ("fp64" if d["fp64"] else "bf16") + (
+            "/pol" if d["polarized"] else "/unpol")
+        work = (d["ntime"] * d["nfreq"]
+                * (d["nant"] * (2 if d["polarized"] else 1)) ** 2 * d["nsrc"])
         reqs.append((float(arrivals[i]), kw, label, work))
     return reqs
 
@@ -597,6 +578,69 @@ def measured_traffic(form: str, nant: int, nsrc: int):
     return None if rec is None else float(rec["bytes_per_launch"])
 
 
+def _kernel_key(name: str) -> str:
+    """`ns::kernel<args>` from a demangled kernel name (parameters dropped)."""
+    name = name.replace("(anonymous namespace)::", "")
+    depth = 0
+    for i, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            name = name[:i]
+            break
+    name = name.strip()
+    return name[5:] if name.startswith("void ") else name
+
+
+def summarize_counters(fetch_dir: str, write_dir: str) -> dict:
+    """Per-kernel HBM traffic from two rocprofv3 counter runs of the same
+    command (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch): means over the
+    dispatches of each kernel, FETCH_SIZE and WRITE_SIZE each scaled by what
+    the counter reads for the 1 GiB device copy of --copy-probe, and the
+    mean dispatch duration of the counter runs (End - Start timestamps)."""
+    import csv
+    import glob
+
+    def load(d, counter):
+        per = {}
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"),
+                           recursive=True):
+            for r in csv.DictReader(open(f)):
+                if r["Counter_Name"] != counter:
+                    continue
+                k = _kernel_key(r["Kernel_Name"])
+                dur = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+                per.setdefault(k, []).append(
+                    (float(r["Counter_Value"]) * 1024.0, dur))
+        return per
+
+    fetch, write = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    gib = float(1 << 30)
+    probe = [k for k in write if "copyBuffer" in k]
+    if not probe:
+        raise RuntimeError("no copy-probe dispatch in the counter runs")
+    # the probe is the copy whose writes are largest (1 GiB)
+    pw = max(v for k in probe for v, _ in write[k])
+    pf = max(v for k in probe for v, _ in fetch.get(k, [(0.0, 0)]))
+    fs, ws = gib / pf, gib / pw
+    out = {"_calibration": {
+        "fetch_scale": fs, "write_scale": ws,
+        "probe": "one 1 GiB device-to-device copy (bench.py --copy-probe)"}}
+    for k in sorted(set(fetch) & set(write)):
+        if "copyBuffer" in k:
+            continue
+        rb = fs * float(np.mean([v for v, _ in fetch[k]]))
+        wb = ws * float(np.mean([v for v, _ in write[k]]))
+        us = float(np.mean([d for _, d in fetch[k] + write[k]])) / 1e3
+        out[k] = {"dispatches": len(fetch[k]), "read_bytes": rb,
+                  "write_bytes": wb, "bytes_per_launch": rb + wb,
+                  "mean_us_counter_runs": us,
+                  "gb_per_s": (rb + wb) / (us * 1e3)}
+    return out
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -657,10 +701,19 @@ def main():
                     help="seconds after the window for requests to finish")
     ap.add_argument("--no-diffuse", action="store_true",
                     help="skip the diffuse-sky (HEALPix nside 128) run")
+    ap.add_argument("--summarize-counters", nargs=2,
+                    metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="print per-kernel HBM traffic (JSON) from two "
+                    "rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of a "
+                    "command that included --copy-probe, and exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy first (the "
                          "known-size kernel that calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.summarize_counters:
+        print(json.dumps(summarize_counters(*args.summarize_counters),
+                         indent=1))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: matvis_amd has no multi-GPU path "
               "(instances shard by (time, frequency) across GPUs)",

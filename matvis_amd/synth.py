@@ -148,3 +148,63 @@ def harmonic_efield_beam(seed: int, naz: int = 360, nza: int = 91,
         data *= 1.0 + perturb * prng.uniform(-1, 1, size=data.shape)
     return GridBeam(data.astype(dtype), az, za, freq_array=np.array([150e6]),
                     beam_type="efield")
+
+
+#: Antenna counts of the serving workload's requests.
+SERVE_NANTS = (32, 128, 350, 512)
+
+
+def serving_draw(rng) -> dict:
+    """Sizes of one request of the Poisson serving workload: nant from
+    {32, 128, 350, 512}, nsrc log-uniform on [1e3, 2e5], nfreq uniform on
+    [1, 64], ntime uniform on [1, 16], polarized with p = 0.5, plain FP64 or
+    split-BF16 with p = 0.5 (drawn in that order from ``rng``)."""
+    return dict(nant=int(rng.choice(SERVE_NANTS)),
+                nsrc=int(np.exp(rng.uniform(np.log(1e3), np.log(2e5)))),
+                nfreq=int(rng.integers(1, 65)),
+                ntime=int(rng.integers(1, 17)),
+                polarized=bool(rng.random() < 0.5),
+                fp64=bool(rng.random() < 0.5))
+
+
+class ServingSky:
+    """Shared inputs of the serving workload: one harmonic E-field beam on
+    the 1-degree 360 x 91 grid (bicubic), 16 times 2 min apart at the HERA
+    site with their rotation matrices, hex arrays with 14.6 m spacing."""
+
+    def __init__(self, beam=None):
+        from . import astro
+
+        self.loc = hera_location()
+        self.beam = beam or harmonic_efield_beam(seed=4, naz=360, nza=91)
+        self.times = times_spaced(16, 2.0)
+        self.rots = astro.rotation_matrices(self.times, self.loc)
+        self._ants: dict[int, np.ndarray] = {}
+
+    def antpos(self, nant: int) -> np.ndarray:
+        if nant not in self._ants:
+            self._ants[nant] = hex_array(nant, 14.6, 0.02, seed=nant)
+        return self._ants[nant]
+
+    def request(self, nant: int, nsrc: int, nfreq: int, ntime: int,
+                polarized: bool, fp64: bool, seed: int,
+                flux_scale: float = 1.0) -> dict:
+        """``simulate_multi`` keywords of one request: RA uniform, sin(dec)
+        uniform on [-90, +30] deg, power-law S^-2 fluxes on [0.1, 10] Jy
+        (times ``flux_scale``), nfreq frequencies evenly over 100-200 MHz,
+        the first ntime times."""
+        from . import astro
+
+        ra, dec = random_sky(nsrc, seed, -90.0, 30.0)
+        flux = powerlaw_fluxes(nsrc, seed + 1) * flux_scale
+        freqs = np.linspace(100e6, 200e6, nfreq)
+        return dict(antpos=self.antpos(nant), freqs=freqs,
+                    times=self.times[:ntime],
+                    skycoords=astro.SkyCoord(ra, dec), telescope_loc=self.loc,
+                    fluxes=np.repeat(flux[:, None], nfreq, 1),
+                    beam_list=[self.beam], polarized=polarized, precision=2,
+                    coord_method="CoordinateRotationMatrix",
+                    coord_method_params={"rotations": self.rots[:ntime]},
+                    beam_spline_opts={"order": 3},
+                    matprod_method="GPUMatMul" if fp64
+                    else "GPUMatMulSplitBF16")
