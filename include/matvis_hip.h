@@ -203,9 +203,12 @@ int mv_zmat_parts(const void *sqrt_flux, const void *beam,
  * compaction's global source ids) is non-NULL, else as flux[s].
  * beam: (nbeam, nfeed, nax, ld_beam) complex (beam_is_f64).
  * form: MV_FORM_*; z0..z2 are the output arrays: z0 the complex matrix for
- * plain forms; for split forms the hi/mid/lo term arrays (nterms of them) of
- * interleaved (re, im) 16-bit values.  Layout of every output: row r at
- * r*ldz (units of complex elements), axis segment ax at ax*seg_stride.
+ * plain forms; for split forms the hi/mid/lo term arrays (nterms of them),
+ * 32 bits per complex element: each 16-byte chunk of 4 consecutive columns
+ * holds the 4 real parts (16 bits each) and then the 4 imaginary parts
+ * (split forms need ldz and seg_stride multiples of 4, so that rows and
+ * segments start on chunks).  Layout of every output: row r at r*ldz
+ * (units of complex elements), axis segment ax at ax*seg_stride.
  * Columns s in [n, ceil(n/pad)*pad) are zero-filled (pad = 32).
  * scale: multiplies Z before the split (a power of two; 1 for plain). */
 int mv_zmat_fused(const double *antpos, int nant, double freq,
@@ -219,7 +222,10 @@ int mv_zmat_fused(const double *antpos, int nant, double freq,
 
 /* Split a complex matrix (c128 or c64, layout as mv_zmat_fused output) into
  * nterms 16-bit terms of `form` (SPLIT_BF16 / SPLIT_F16), after multiplying
- * by `scale`.  Rows [0, rows), columns [0, cols) of each of nseg segments.
+ * by `scale`, in the chunked term layout of mv_zmat_fused (ldt and, with
+ * nseg > 1, tseg_stride multiples of 4).  Rows [0, rows), columns
+ * [0, cols) of each of nseg segments; the slots of a segment's last chunk
+ * past cols are written as zeros.
  * For Z built outside mv_zmat_fused, e.g. by the plugin chain's ZMatrixCalc
  * (src/matvis/core/getz.py:80-100) ahead of a split MatProd. */
 int mv_split_z(const void *z, int z_is_f64, int64_t rows, int nseg,

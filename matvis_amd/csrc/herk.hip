@@ -19,7 +19,9 @@
 // the row of Z read as reals, and Y_r = (im_0, -re_0, im_1, -re_1, ...),
 //   Re V[r, c] = X_r . X_c        Im V[r, c] = X_r . Y_c
 // so each output tile needs two real products over 2K reals: 8 flop per
-// complex multiply-add, with no extra passes over Z.
+// complex multiply-add, with no extra passes over Z.  (The split forms'
+// terms order the reals of each 4 columns as re_0..re_3, im_0..im_3: the dot
+// products are the same sums, and Y is a register reorder of X.)
 #include <map>
 #include <mutex>
 #include <utility>
@@ -325,13 +327,12 @@ template <> struct SplitMfma<true> {
   }
 };
 
-// (re, im) -> (im, -re) on a packed pair of 16-bit floats.
-__device__ __forceinline__ uint32_t swap_negate(uint32_t w) {
-  return ((w >> 16) | (w << 16)) ^ 0x80000000u;
-}
+// The imaginary-part operand Y of a 16-byte chunk of 4 columns in the split
+// terms' layout (the 4 real parts, then the 4 imaginary parts, 16 bits
+// each): (re x 4 | im x 4) -> (im x 4 | -re x 4), a register reorder and two
+// sign flips of packed pairs.
 __device__ __forceinline__ u32x4 swap_negate4(u32x4 v) {
-  return u32x4{swap_negate(v[0]), swap_negate(v[1]), swap_negate(v[2]),
-               swap_negate(v[3])};
+  return u32x4{v[2], v[3], v[0] ^ 0x80008000u, v[1] ^ 0x80008000u};
 }
 
 
@@ -351,7 +352,9 @@ template <bool F16> __device__ __forceinline__ float hi_half(uint32_t w) {
 }
 
 // sum over the 8 complex values a thread stages of |t0 + t1 (+ t2)|^2, in
-// FP64 (two bf16 terms add exactly in FP32; three round once at 2^-24)
+// FP64 (two bf16 terms add exactly in FP32; three round once at 2^-24): the
+// sum of the squares of all 16 summed components, whichever column each
+// 16-bit half of a word belongs to
 template <int NT, bool F16>
 __device__ __forceinline__ double row_norm2(const u32x4 (*r)[2]) {
   double s = 0.0;

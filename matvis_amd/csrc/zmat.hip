@@ -124,24 +124,39 @@ template <> struct ZWriter<MV_FORM_C64, 1> {
     ((cplx<float> *)zs[0])[off] = v;
   }
 };
+// Split forms: per term, 16-byte chunks of 4 consecutive columns, the real
+// parts of the 4 columns (16 bits each) in the chunk's first 8 bytes and
+// their imaginary parts in its last 8.  Column c of a row is slot c % 4 of
+// chunk c / 4 (rows and segments start on chunk boundaries).  Lanes 4q ..
+// 4q + 3 hold the 4 columns of one chunk; each lane trades a half-word with
+// its xor-1 partner and stores one whole word: even slots the word of real
+// parts, odd slots the word of imaginary parts.  Every lane of a quad must
+// call put() (the exchange is a wave operation).
+template <bool F16, int NT>
+__device__ __forceinline__ void put_split(void *const *zs, int64_t off,
+                                          cplx<float> v) {
+  uint32_t r[NT], i[NT];
+  split_terms<F16, NT>(v.re, r);
+  split_terms<F16, NT>(v.im, i);
+  const int j = (int)(off & 3);
+  const bool odd = j & 1;
+  const int64_t w = off - j + (odd ? 2 : 0) + (j >> 1);
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    const uint32_t recv = (uint32_t)__shfl_xor((int)(odd ? r[k] : i[k]), 1, 64);
+    ((uint32_t *)zs[k])[w] = odd ? (recv | (i[k] << 16)) : (r[k] | (recv << 16));
+  }
+}
 template <int NT> struct ZWriter<MV_FORM_SPLIT_BF16, NT> {
   typedef float T;
   __device__ static void put(void *const *zs, int64_t off, cplx<float> v) {
-    uint32_t r[NT], i[NT];
-    split_terms<false, NT>(v.re, r);
-    split_terms<false, NT>(v.im, i);
-#pragma unroll
-    for (int k = 0; k < NT; ++k) ((uint32_t *)zs[k])[off] = r[k] | (i[k] << 16);
+    put_split<false, NT>(zs, off, v);
   }
 };
 template <int NT> struct ZWriter<MV_FORM_SPLIT_F16, NT> {
   typedef float T;
   __device__ static void put(void *const *zs, int64_t off, cplx<float> v) {
-    uint32_t r[NT], i[NT];
-    split_terms<true, NT>(v.re, r);
-    split_terms<true, NT>(v.im, i);
-#pragma unroll
-    for (int k = 0; k < NT; ++k) ((uint32_t *)zs[k])[off] = r[k] | (i[k] << 16);
+    put_split<true, NT>(zs, off, v);
   }
 };
 
@@ -169,42 +184,49 @@ __global__ __launch_bounds__(kThreads) void zmat_fused_kernel(
   if (s >= n_pad || s >= seg_stride) return;
   const int a0 = blockIdx.y * ants_per_block;
   const int a1 = min(a0 + ants_per_block, nant);
-  if (s >= n) {
-    const cplx<T> zero = {(T)0, (T)0};
-    for (int a = a0; a < a1; ++a)
-      for (int fd = 0; fd < nf; ++fd)
-        for (int ax = 0; ax < na; ++ax)
-          ZWriter<FORM, NT>::put(zo.z, ((int64_t)a * nf + fd) * ldz +
-                                           (int64_t)ax * seg_stride + s,
-                                 zero);
-    return;
+  // lanes past the column count write zeros; every lane reaches the writes
+  // (the split forms' writes are wave operations)
+  const bool valid = s < n;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  T amp = (T)0;
+  if (valid) {
+    sx = crd[s];
+    sy = crd[ld_crd + s];
+    sz = crd[2 * ld_crd + s];
+    amp = (T)(flux[idx ? (int64_t)idx[s] : s] * scale);
   }
-  const double sx = crd[s], sy = crd[ld_crd + s], sz = crd[2 * ld_crd + s];
-  const T amp = (T)(flux[idx ? (int64_t)idx[s] : s] * scale);
   int cached = -1;
   cplx<T> A[nf * na];
   for (int a = a0; a < a1; ++a) {
-    const int b = beam_idx ? beam_idx[a] : (nbeam == 1 ? 0 : a);
-    if (b != cached) {
-      // an index outside the beam list is bad input: NaN, never an
-      // out-of-range read
-      const bool valid = (unsigned)b < (unsigned)nbeam;
+    cplx<T> out[nf * na];
 #pragma unroll
-      for (int q = 0; q < nf * na; ++q)
-        A[q] = valid ? beam[((int64_t)b * nf * na + q) * ld_beam + s]
-                     : cplx<T>{(T)NAN, (T)NAN};
-      cached = b;
+    for (int q = 0; q < nf * na; ++q) out[q] = cplx<T>{(T)0, (T)0};
+    if (valid) {
+      const int b = beam_idx ? beam_idx[a] : (nbeam == 1 ? 0 : a);
+      if (b != cached) {
+        // an index outside the beam list is bad input: NaN, never an
+        // out-of-range read
+        const bool ok = (unsigned)b < (unsigned)nbeam;
+#pragma unroll
+        for (int q = 0; q < nf * na; ++q)
+          A[q] = ok ? beam[((int64_t)b * nf * na + q) * ld_beam + s]
+                    : cplx<T>{(T)NAN, (T)NAN};
+        cached = b;
+      }
+      cplx<T> e =
+          Phasor<T>::eval(reduced_turns(antpos + 3 * a, fc, sx, sy, sz));
+      e.re *= amp;
+      e.im *= amp;
+#pragma unroll
+      for (int q = 0; q < nf * na; ++q) out[q] = cmul(A[q], e);
     }
-    cplx<T> e = Phasor<T>::eval(reduced_turns(antpos + 3 * a, fc, sx, sy, sz));
-    e.re *= amp;
-    e.im *= amp;
 #pragma unroll
     for (int fd = 0; fd < nf; ++fd)
 #pragma unroll
       for (int ax = 0; ax < na; ++ax)
         ZWriter<FORM, NT>::put(
             zo.z, ((int64_t)a * nf + fd) * ldz + (int64_t)ax * seg_stride + s,
-            cmul(A[fd * na + ax], e));
+            out[fd * na + ax]);
   }
 }
 
@@ -215,10 +237,14 @@ __global__ __launch_bounds__(kThreads) void split_kernel(
     int64_t tseg_stride) {
   int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   int64_t rs = blockIdx.y;  // row * nseg + seg
-  if (c >= cols) return;
+  // whole chunks of 4 columns: the slots past `cols` get zeros
+  if (c >= cdiv(cols, (int64_t)4) * 4) return;
   int64_t r = rs / nseg, sg = rs % nseg;
-  cplx<TI> v = z[r * ldz + sg * seg_stride + c];
-  cplx<float> w = {(float)v.re * scale, (float)v.im * scale};
+  cplx<float> w = {0.0f, 0.0f};
+  if (c < cols) {
+    const cplx<TI> v = z[r * ldz + sg * seg_stride + c];
+    w = cplx<float>{(float)v.re * scale, (float)v.im * scale};
+  }
   ZWriter<FORM, NT>::put(to.z, r * ldt + sg * tseg_stride + c, w);
 }
 
@@ -324,6 +350,10 @@ extern "C" int mv_zmat_fused(const double *antpos, int nant, double freq,
              (long long)seg_stride, kZPad);
   MV_REQUIRE(ldz >= (int64_t)nax * seg_stride,
              "mv_zmat_fused: ldz must be >= nax * seg_stride");
+  MV_REQUIRE((form != MV_FORM_SPLIT_BF16 && form != MV_FORM_SPLIT_F16) ||
+                 (ldz % 4 == 0 && seg_stride % 4 == 0),
+             "mv_zmat_fused: split terms need ldz and seg_stride multiples "
+             "of 4 (4-column chunks)");
   const bool plain = form == MV_FORM_C128 || form == MV_FORM_C64;
   MV_REQUIRE(plain || form == MV_FORM_SPLIT_BF16 || form == MV_FORM_SPLIT_F16,
              "mv_zmat_fused: unknown form %d", form);
@@ -373,6 +403,9 @@ extern "C" int mv_split_z(const void *z, int z_is_f64, int64_t rows, int nseg,
                  seg_stride >= cols && ldz >= nseg * seg_stride &&
                  tseg_stride >= cols && ldt >= nseg * tseg_stride,
              "mv_split_z: bad sizes");
+  MV_REQUIRE(ldt % 4 == 0 && (nseg == 1 || tseg_stride % 4 == 0),
+             "mv_split_z: term rows and segments must start on 4-column "
+             "chunks (ldt and tseg_stride multiples of 4)");
   if (rows == 0 || cols == 0) return MV_OK;
   dim3 g((unsigned)cdiv(cols, kThreads), (unsigned)(rows * nseg));
   hipStream_t st = as_stream(stream);

@@ -110,7 +110,8 @@ class VisibilityEngine:
         nA = nb if shared else 1
         self.A = [D.zeros((self.nbeam, self.nfeed, self.nax, seg),
                           self.beam_ctype) for _ in range(nA)]
-        # Z: (m, nax, seg) complex, or nterms arrays of packed 16-bit pairs
+        # Z: (m, nax, seg) complex, or nterms arrays of 16-bit terms (32 bits
+        # a complex element, 16-byte chunks of 4 columns: re x 4, im x 4)
         self.ldz = self.nax * seg
         if is_split(self.form):
             self.z = [D.zeros((self.m, self.ldz), np.int32)
