@@ -288,6 +288,22 @@ int mv_herk(int form, int nterms, int nprod, const void *z0, const void *z1,
             double scale, double *vacc, int64_t ldv, int accumulate,
             void *workspace, size_t ws_bytes, mv_stream_t stream);
 
+/* mv_herk with no workspace argument and no allocation: the split-K
+ * partials go to a 128 MiB device array that the library holds from load
+ * until the process ends (one copy on each device it runs on).  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  Uses as many K-slices as the array holds, at most
+ * mv_herk's count; m whose partials for one slice exceed the array (split
+ * forms m > 5632, complex128 m > 4032, complex64 m > 5760) is rejected with
+ * MV_ERR_CAPACITY.  Calls that overlap on one device share the array and
+ * race: order them (one stream, or events).  Other arguments as mv_herk. */
+int mv_herk_noalloc(int form, int nterms, int nprod, const void *z0,
+                    const void *z1, const void *z2, int m, int64_t ldz,
+                    int nseg, int64_t seg_stride, int64_t seg_len,
+                    const int32_t *seg_len_dev, double scale, double *vacc,
+                    int64_t ldv, int accumulate, mv_stream_t stream);
+
 /* mv_herk in its two launches, same arguments: the MFMA kernel that writes
  * the slice partials to the workspace, and the FP64 reduce into Vacc.  A
  * caller can record events between them (the compute kernel's own duration)

@@ -58,6 +58,8 @@ SIGNATURES: dict[str, list] = {
     "mv_herk_slices": [_I, _I, _I, _I, _I, _L, _I],
     "mv_herk": [_I, _I, _I, _P, _P, _P, _I, _L, _I, _L, _L, _P, _I, _D, _P,
                 _L, _I, _P, _S, _P],
+    "mv_herk_noalloc": [_I, _I, _I, _P, _P, _P, _I, _L, _I, _L, _L, _P, _D,
+                        _P, _L, _I, _P],
     "mv_herk_compute": [_I, _I, _I, _P, _P, _P, _I, _L, _I, _L, _L, _P, _I,
                         _D, _P, _L, _I, _P, _S, _P],
     "mv_herk_reduce": [_I, _I, _I, _P, _P, _P, _I, _L, _I, _L, _L, _P, _I,

@@ -1280,6 +1280,79 @@ extern "C" int mv_herk_reduce(MV_HERK_ARGS) {
 #undef MV_HERK_ARGS
 #undef MV_HERK_PASS
 
+// ------------------------------------------------------------ no-alloc entry
+
+namespace {
+// The split-K partials of mv_herk_noalloc: device memory the library holds
+// from load until the process ends, one copy on each device it runs on
+// (128 MiB: room for 48 slices of the 350-antenna polarized split form,
+// more than its accuracy cap of 192 K tiles a slice needs at 100k sources).
+// Host code uses the per-device address from hipGetSymbolAddress, never the
+// array's name.  Every call on a device shares that device's copy and
+// leaves its partials in it; nothing in it needs clearing (each call's
+// kernels write every partial they later read).
+constexpr size_t kScratchBytes = size_t(128) << 20;
+}  // namespace
+
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr;
+
+// The address of this device's scratch copy (looked up once per device).
+int scratch_address(void **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+    return set_error(MV_ERR_HIP, "mv_herk_noalloc: hipGetDevice failed");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto f = g_scratch_ptr.find(dev);
+  if (f != g_scratch_ptr.end()) {
+    *out = f->second;
+    return MV_OK;
+  }
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return set_error(MV_ERR_HIP,
+                     "mv_herk_noalloc: hipGetSymbolAddress(scratch): %s",
+                     hipGetErrorString(e));
+  g_scratch_ptr[dev] = ptr;
+  *out = ptr;
+  return MV_OK;
+}
+}  // namespace
+
+extern "C" int mv_herk_noalloc(int form, int nterms, int nprod,
+                               const void *z0, const void *z1, const void *z2,
+                               int m, int64_t ldz, int nseg,
+                               int64_t seg_stride, int64_t seg_len,
+                               const int32_t *seg_len_dev, double scale,
+                               double *vacc, int64_t ldv, int accumulate,
+                               mv_stream_t stream) {
+  const int ts = mv_herk_tile(form);
+  MV_REQUIRE(ts > 0, "mv_herk_noalloc: unknown form %d", form);
+  MV_REQUIRE(m >= 1 && nseg >= 1 && seg_len >= 0,
+             "mv_herk_noalloc: bad sizes m=%d nseg=%d seg_len=%lld", m, nseg,
+             (long long)seg_len);
+  // the slices the scratch holds, at most what mv_herk would size for
+  const size_t one = mv_herk_workspace_bytes(form, m, 1);
+  if (one == 0 || one > kScratchBytes)
+    return set_error(MV_ERR_CAPACITY,
+                     "mv_herk_noalloc: m=%d needs %zu bytes of partials for "
+                     "one K-slice, more than the %zu-byte scratch",
+                     m, one, kScratchBytes);
+  int ns = mv_herk_max_slices(form, m, (int64_t)nseg * seg_len);
+  const int fit = (int)(kScratchBytes / one);
+  ns = ns < 1 ? 1 : (ns > fit ? fit : ns);
+  void *ws = nullptr;
+  if (int e = scratch_address(&ws)) return e;
+  return herk_run(kPhaseCompute | kPhaseReduce, form, nterms, nprod, z0, z1,
+                  z2, m, ldz, nseg, seg_stride, seg_len, seg_len_dev, ns,
+                  scale, vacc, ldv, accumulate, ws,
+                  mv_herk_workspace_bytes(form, m, ns), stream);
+}
+
 // ------------------------------------------------------------ grouped API
 
 namespace {

@@ -21,6 +21,7 @@
 #define __shared__ static
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define __align__(n) __attribute__((aligned(n)))
 
 struct dim3 {
   unsigned x, y, z;
@@ -38,6 +39,13 @@ inline hipError_t hipGetLastError() { return hipSuccess; }
 inline const char *hipGetErrorString(hipError_t) { return "emulated"; }
 inline hipError_t hipGetDevice(int *d) {
   *d = 0;
+  return hipSuccess;
+}
+// A file-scope __device__ array is ordinary host memory here, so its symbol
+// address is the array's own address.
+#define HIP_SYMBOL(sym) (reinterpret_cast<const void *>(&(sym)))
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) {
+  *p = const_cast<void *>(sym);
   return hipSuccess;
 }
 // A small emulated device: 3 CUs x 2 blocks = 6 slots, rounded by callers to
