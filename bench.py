@@ -37,6 +37,7 @@ NFREQ = 20
 NTIMES_POOL = 30
 MFMA_BF16_PEAK = 2516.6e12
 FP64_PEAK = 78.6e12
+HBM_PEAK = 8.0e12
 #: dense MFMA peaks of MI355X (TFLOP/s) for each form's arithmetic
 PEAK_TFLOPS = {"FORM_SPLIT_BF16": 2516.6, "FORM_SPLIT_F16": 2516.6,
                "FORM_C128": 78.6, "FORM_C64": 157.3}
@@ -565,17 +566,62 @@ def blas_threads() -> int:
     return os.cpu_count() or 1
 
 
-def measured_traffic(form: str, nant: int, nsrc: int):
-    """HBM bytes per HERK launch from the committed counter summary
-    (profiles/herk_traffic.json: FETCH_SIZE and WRITE_SIZE per dispatch,
-    each scaled by what the counter reads for a 1 GiB copy), when it covers
-    this form and workload; None otherwise."""
-    path = ROOT / "profiles" / "herk_traffic.json"
+TRAFFIC = ROOT / "profiles" / "hbm_traffic_350ant_100ksrc.json"
+
+#: the kernels of one headline step (one time, every frequency) in the
+#: committed counter summary: (stage, kernel key, launches per step)
+STAGE_KERNELS = [
+    ("rotate+cull: count", "mv::cull_count_kernel", 1),
+    ("rotate+cull: scan", "mv::cull_scan_kernel", 1),
+    ("rotate+cull: scatter", "mv::cull_scatter_kernel<double>", 1),
+    ("beam interpolation", "mv::interp_kernel<float, double, 3, false>", 1),
+    ("fused Z (split-BF16 terms)", "mv::zmat_fused_kernel<2, 2, true>",
+     "nfreq"),
+    ("HERK (split-BF16)", "mv::herk_split_kernel<2, 1, false, 128>",
+     "nfreq"),
+    ("split-K reduce", "mv::herk_reduce_kernel<float>", "nfreq"),
+]
+
+
+def _traffic(nant: int, nsrc: int):
+    """The committed counter summary (FETCH_SIZE and WRITE_SIZE per
+    dispatch, each scaled by what the counter reads for a 1 GiB copy) when
+    it was measured on this workload; None otherwise."""
     try:
-        rec = json.loads(path.read_text()).get(f"{form}:{nant}x{nsrc}")
+        rec = json.loads(TRAFFIC.read_text())
     except (OSError, ValueError):
         return None
-    return None if rec is None else float(rec["bytes_per_launch"])
+    wl = rec.get("_workload", {})
+    if wl.get("nant") != nant or wl.get("nsrc") != nsrc:
+        return None
+    return rec
+
+
+def measured_traffic(kernel: str, nant: int, nsrc: int):
+    """HBM bytes per launch of `kernel` from the counter summary, or None."""
+    rec = _traffic(nant, nsrc)
+    if rec is None or kernel not in rec:
+        return None
+    return float(rec[kernel]["bytes_per_launch"])
+
+
+def stage_traffic(nant: int, nsrc: int, nfreq: int):
+    """Counter-measured HBM bytes and GB/s of every kernel in one step
+    (durations: the same dispatches' own timestamps in the counter runs)."""
+    rec = _traffic(nant, nsrc)
+    if rec is None:
+        return None
+    out = {}
+    for stage, k, per in STAGE_KERNELS:
+        if k not in rec:
+            continue
+        r = rec[k]
+        out[stage] = {
+            "kernel": k, "launches_per_step": nfreq if per == "nfreq" else per,
+            "read_bytes": r["read_bytes"], "write_bytes": r["write_bytes"],
+            "us": r["mean_us_counter_runs"], "gb_per_s": r["gb_per_s"],
+            "frac_of_hbm_peak": r["gb_per_s"] * 1e9 / HBM_PEAK}
+    return out
 
 
 def _kernel_key(name: str) -> str:
@@ -776,7 +822,7 @@ def main():
             / PEAK_TFLOPS[FORMS[name][0]],
             n_up_mean=float(np.mean(nups)),
         )
-        if name == forms[0]:
+        if FORMS[name][0] == "FORM_SPLIT_BF16" or name == forms[0]:
             results[name]["host_output"] = host_output_rate(
                 eng, min(max(steps, 4), NTIMES_POOL), nfreq * vis_per_inst)
         # keep one instance (time 0, first frequencies) for the error figures
@@ -802,6 +848,8 @@ def main():
         for name in outs:
             err[f"{name}_vs_oracle"] = nerr(outs[name], ref)
         cpu = {"value": cpu_vis, "unit": "vis/s", "cores": cores,
+               "host_cpus": os.cpu_count(),
+               "affinity_cpus": len(os.sched_getaffinity(0)),
                "kind": "port",
                "sample": "time 0 x first 2 frequencies of the same workload: "
                          "2 instances through the numpy oracle (BLAS ZGEMM "
@@ -848,14 +896,22 @@ def main():
         "meets_1e-6": bool(head_name in ok) if head_name != "fp64" else None,
         "p50_ms": head["p50_ms"],
         "p99_ms": head["p99_ms"],
+        # the drop-in API's rate with the visibilities copied back into the
+        # caller's host arrays (PCIe and host copies included)
+        "value_host_delivered": head.get("host_output", {}).get("vis_per_s"),
         "roofline": {"bound": "mfma", "achieved": achieved,
                      "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / (peak / 1e12),
-                     "traffic": measured_traffic(head_name, nant, args.nsrc),
+                     "traffic": measured_traffic(
+                         "mv::herk_split_kernel<2, 1, false, 128>"
+                         if head_name == "bf16"
+                         else "mv::herk_plain_kernel<double>"
+                         if head_name == "fp64" else "", nant, args.nsrc),
                      "kernel": "herk_split_kernel" if is_split
                      else "herk_plain_kernel<double>",
                      "useful_tflops": head["herk_tflops"]},
         "cpu_baseline": cpu,
+        "stages": stage_traffic(nant, args.nsrc, nfreq),
         "forms": results,
         "errors_normalised_max_median_elemmax_elemmedian": err,
     }

@@ -259,3 +259,29 @@ def test_product_error_metric_matches_oracle(nf):
     # element (2, 5, 0, nf-1) pairs conj(Z[2, nf-1]) with Z[5, 0]
     np.testing.assert_allclose(a[0], 1e-3 / np.sqrt(auto(2, nf - 1) * auto(5, 0)),
                                rtol=1e-12)
+
+
+def test_data_path_resolution(tmp_path, monkeypatch):
+    """DATA_PATH: $MATVIS_DATA_PATH, else an installed matvis package's
+    data directory (found without importing it), else matvis_amd/data."""
+    import sys
+
+    import matvis_amd
+
+    monkeypatch.setenv("MATVIS_DATA_PATH", str(tmp_path))
+    assert matvis_amd._data_path() == tmp_path
+    monkeypatch.delenv("MATVIS_DATA_PATH")
+    pkg = tmp_path / "site" / "matvis"
+    (pkg / "data").mkdir(parents=True)
+    (pkg / "__init__.py").write_text("raise RuntimeError('imported')\n")
+    monkeypatch.syspath_prepend(str(tmp_path / "site"))
+    assert matvis_amd._data_path() == pkg / "data"
+    assert "matvis" not in sys.modules
+    monkeypatch.setattr(sys, "path", [p for p in sys.path
+                                      if p != str(tmp_path / "site")])
+    import importlib
+
+    importlib.invalidate_caches()
+    got = matvis_amd._data_path()
+    assert got.name == "data" and got.parent.name in ("matvis_amd", "matvis")
+
