@@ -34,6 +34,15 @@ import math
 import numpy as np
 
 ARCSEC = math.pi / (180.0 * 3600.0)
+
+#: measured distance of the built-in context's apparent directions from
+#: ERFA's ``eraAtco13`` (arcsec; 48 sources x 5 dates at the HERA site)
+BUILTIN_CONTEXT_ERROR_ARCSEC = (0.05, 0.09)
+
+
+class AstrometryAccuracyWarning(UserWarning):
+    """The built-in astrometry model is in use: apparent directions are
+    0.05-0.09 arcsec from ERFA's, not within the reference's 10 mas."""
 J2000 = 2451545.0
 TT_MINUS_UTC_S = 69.184  # TAI-UTC (37 s) + 32.184 s, valid from 2017
 

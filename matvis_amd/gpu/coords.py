@@ -15,8 +15,10 @@ context comes from, in order: ``astrom`` (caller-supplied, one dict per
 time with the fields of ERFA's ``eraApco13``: eh, em, v, bm1, bpn, eral,
 xpl, ypl, sphi, cphi), astropy's ``erfa_astrom`` when the times are astropy
 ``Time`` objects (what the reference calls, cpu/coords.py:165-166), else the
-host model :func:`matvis_amd.astro.erfa_context` (within 0.1 arcsec of
-ERFA's full chain; UT1 = UTC, no polar motion).
+host model :func:`matvis_amd.astro.erfa_context` (0.05-0.09 arcsec from
+ERFA's full chain against the reference's 10 mas; UT1 = UTC, no polar
+motion), which warns once per rotation object with
+:class:`matvis_amd.astro.AstrometryAccuracyWarning`.
 """
 
 from __future__ import annotations
@@ -25,7 +27,10 @@ import numpy as np
 
 from .. import _device as D
 from .._lib import call
-from ..astro import (erfa_context, location_lat_lon, skycoord_ra_dec,
+import warnings
+
+from ..astro import (BUILTIN_CONTEXT_ERROR_ARCSEC, AstrometryAccuracyWarning,
+                     erfa_context, location_lat_lon, skycoord_ra_dec,
                      times_jd)
 from ..core.coords import CoordinateRotation
 
@@ -85,6 +90,7 @@ class CoordinateRotationERFA(CoordinateRotation):
             raise ValueError("astrom needs one context per time")
         self._astrom = astrom
         self._time_of_last_evaluation = None
+        self._warned_builtin = False
 
     def setup(self):
         super().setup()
@@ -99,6 +105,15 @@ class CoordinateRotationERFA(CoordinateRotation):
         ctx = _astropy_context(self.times[t], self.telescope_loc)
         if ctx is not None:
             return ctx
+        if not self._warned_builtin:
+            lo, hi = BUILTIN_CONTEXT_ERROR_ARCSEC
+            warnings.warn(
+                "no astrometry context from astropy or the caller: using the "
+                f"built-in model, {1e3 * lo:.0f}-{1e3 * hi:.0f} mas from "
+                "ERFA's eraAtco13 (the reference's ERFA path is within 10 "
+                "mas); pass astropy Time objects or astrom= contexts for "
+                "full accuracy", AstrometryAccuracyWarning, stacklevel=2)
+            self._warned_builtin = True
         lat, lon = location_lat_lon(self.telescope_loc)
         height = getattr(self.telescope_loc, "height", 0.0)
         height = float(height.to_value("m") if hasattr(height, "to_value")

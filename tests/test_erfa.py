@@ -243,3 +243,29 @@ def test_gpu_erfa_precision1_and_chunk_sizes():
             parts.append(np.concatenate(cols, axis=1))
         for p in parts:
             np.testing.assert_array_equal(p, whole)
+
+
+def test_builtin_context_warns_once():
+    """The built-in host model is 0.05-0.09 arcsec from ERFA: using it warns
+    (once per rotation object); caller-supplied contexts do not."""
+    import warnings
+
+    from matvis_amd import astro
+    from matvis_amd.gpu.coords import CoordinateRotationERFA
+
+    lon, lat, hm = G["site"]
+    loc = astro.EarthLocation.from_geodetic(np.degrees(lon), np.degrees(lat),
+                                            hm)
+    kw = dict(flux=np.ones(G["ra"].size), times=G["utc"], telescope_loc=loc,
+              skycoords=astro.SkyCoord(G["ra"], G["dec"]), precision=2)
+    co = CoordinateRotationERFA(0.0, **kw)
+    with pytest.warns(astro.AstrometryAccuracyWarning, match="mas"):
+        c = co.context(0)
+    np.testing.assert_array_equal(
+        c["bpn"], astro.erfa_context(float(G["utc"][0]), lat, lon,
+                                     hm)["bpn"])
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        co.context(1)  # once per object
+        CoordinateRotationERFA(
+            0.0, astrom=[ctx(i) for i in range(NT)], **kw).context(0)
