@@ -366,7 +366,12 @@ extern "C" int mv_zmat_fused(const double *antpos, int nant, double freq,
   MV_REQUIRE(nterms < 3 || z2, "mv_zmat_fused: missing term array 2");
   if (count == 0) return MV_OK;
   const int apb = 16;
-  dim3 g((unsigned)cdiv(cdiv(count, kZPad) * kZPad, kThreads),
+  // blocks go to the 8 XCDs round-robin by linear index x + y gx: with gx a
+  // multiple of 8, the antenna blocks of one source range share an XCD and
+  // its L2, which then serves their re-reads of the per-source inputs (the
+  // blocks past the column count return at once)
+  dim3 g((unsigned)(cdiv(cdiv(cdiv(count, kZPad) * kZPad, kThreads),
+                         (int64_t)8) * 8),
          (unsigned)cdiv(nant, apb));
   hipStream_t st = as_stream(stream);
   const double fc = freq / kC;
