@@ -206,9 +206,20 @@ def batched_bench(ninst: int = 256, seed: int = 7, reps: int = 5):
         flops += 8.0 * nf * nup * m * (m + 1) / 2
     torch.cuda.synchronize()
     log(f"batched: {ninst} instances built ({flops / 1e12:.1f} Tflop useful)")
-    tune = {}
-    tiles = autotune.choose_tiles(form, nt, npd, probs, report=tune)
-    batch = GroupedBatch(form, nt, npd, probs, tiles).capture()
+    # the tuner on the whole batch, and on its 16 smallest instances (a
+    # batch that does not fill the GPU): the picks differ with the mix
+    work = [p["m"] ** 2 * p["nseg"] * p["seg_len"] for p in probs]
+    small = [probs[i] for i in np.argsort(work)[:16]]
+    t0 = time.perf_counter()
+    plan_small = autotune.tune(form, nt, npd, small, budget_s=5.0)
+    tune_small_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plan = autotune.tune(form, nt, npd, probs, budget_s=10.0)
+    tune_s = time.perf_counter() - t0
+    log(f"batched: tuned in {tune_s:.1f} s: {plan.describe()}")
+    tiles = plan.tiles
+    batch = GroupedBatch(form, nt, npd, probs, plan.tiles,
+                         plan.kchains).capture()
     batch.replay()
     torch.cuda.synchronize()
     e0, e1 = (torch.cuda.Event(enable_timing=True) for _ in range(2))
@@ -290,7 +301,9 @@ def batched_bench(ninst: int = 256, seed: int = 7, reps: int = 5):
         "grouped_graph_plain_c64_ms": t_group64,
         "plain_c64_speedup_grouped_vs_sequential": t_plain / t_group64,
         "tiles": {str(k): tiles.count(k) for k in sorted(set(tiles))},
-        "autotune_ms": tune,
+        "autotune": dict(plan.describe(), seconds=tune_s),
+        "autotune_16_smallest": dict(plan_small.describe(),
+                                     seconds=tune_small_s),
         "max_err_vs_fp64_8_instances": max(errs),
         "median_err_vs_fp64_8_instances": float(np.median(meds)),
     }
@@ -409,15 +422,18 @@ def serving_requests(seconds: float, rate: float, seed: int = 11):
     return reqs
 
 
-def serving_bench(seconds: float = 20.0, rate: float = 20.0,
-                  drain: float = 20.0, seed: int = 11, max_active: int = 16,
-                  policy: str = "fifo"):
+def serving_bench(seconds: float = 60.0, rate: float = 20.0,
+                  drain: float = 30.0, seed: int = 11, max_active: int = 16,
+                  policy: str = "fifo", errors: bool = True):
     """The resident server under a Poisson stream (arrivals for `seconds`,
     then up to `drain` s to finish).  Reports latency of the finished
     requests, lower bounds that count unfinished ones at their age, the
     visibilities delivered per second and, for the two smallest finished
     requests of each class, the error against the same request run alone on
-    the plain FP64 GPU path."""
+    the plain FP64 GPU path (which the GPU tests hold within 1e-12 of the
+    oracle).  Latency trend: the p99 (unfinished requests at their age) of
+    the requests arriving in the first and in the last third of the window;
+    a backlog that grows shows as a last-third p99 well above the first."""
     import threading
 
     import torch
@@ -454,8 +470,11 @@ def serving_bench(seconds: float = 20.0, rate: float = 20.0,
     th = threading.Thread(target=feeder, daemon=True)
     th.start()
     last = t0
+    backlog = []  # (s since start, queued + active) every 0.5 s
     while True:
         now = time.perf_counter()
+        if not backlog or now - t0 - backlog[-1][0] >= 0.5:
+            backlog.append((now - t0, len(srv.queue) + len(srv.active)))
         if th.is_alive() is False and all(r.done.is_set() for r in reqs):
             break
         if now - t0 > seconds + drain:
@@ -476,6 +495,10 @@ def serving_bench(seconds: float = 20.0, rate: float = 20.0,
     # unfinished requests count at their age when the run ended
     ages = np.array([(r.t_done if r.done.is_set() else t_end) - r.t_submit
                      for r in reqs])
+    at = np.array([a for a, _, _, _ in stream])
+
+    def p99_of(mask):
+        return float(np.percentile(ages[mask], 99)) if mask.any() else None
     out = {
         "arrival_window_s": seconds, "rate_per_s": rate, "drain_s": drain,
         "elapsed_s": elapsed, "submitted": len(reqs),
@@ -486,12 +509,21 @@ def serving_bench(seconds: float = 20.0, rate: float = 20.0,
         "p99_s_all_lower_bound": float(np.percentile(ages, 99)),
         "vis_per_s": st["visibilities_all"] / elapsed,
         "finished_per_s": st["requests"] / elapsed,
+        "p99_s_first_third": p99_of(at < seconds / 3),
+        "p99_s_last_third": p99_of(at >= 2 * seconds / 3),
+        "backlog_slope_per_s": backlog_slope(backlog, seconds),
+        "backlog_max": max(b for _, b in backlog),
+        "drain_needed_s": max(0.0, max(
+            (r.t_done - t0 for r in reqs if r.done.is_set()),
+            default=0.0) - seconds),
         "by_class": st.get("by_label", {}),
         "max_active": max_active,
         "policy": policy,
     }
     del srv
     torch.cuda.empty_cache()
+    if not errors:
+        return out
 
     def nerr(got, ref, nant, nf):
         a = got.reshape(-1, nant * nant, nf, nf)
@@ -525,6 +557,48 @@ def serving_bench(seconds: float = 20.0, rate: float = 20.0,
         cls["median"] = float(np.median(cls["median"]))
     out["error_vs_fp64_alone_normalised"] = errs
     return out
+
+
+def backlog_slope(samples, seconds: float) -> float:
+    """Least-squares slope (requests/s) of the backlog (queued + active)
+    over the last two thirds of the arrival window: about zero while the
+    server keeps up, the arrival rate minus the service rate once it does
+    not."""
+    pts = np.array([(t, b) for t, b in samples
+                    if seconds / 3 <= t <= seconds], dtype=np.float64)
+    if len(pts) < 2:
+        return 0.0
+    return float(np.polyfit(pts[:, 0], pts[:, 1], 1)[0])
+
+
+def serving_sweep(rates, seconds: float = 40.0, drain: float = 20.0,
+                  seed: int = 11, policy: str = "fifo") -> dict:
+    """The serving workload at a ladder of Poisson rates under `policy`.  A rate is
+    sustainable when the backlog does not grow (its slope over the last two
+    thirds of the window is under 10% of the arrival rate) and every request
+    finished within the drain; the sweep stops at the first rate that is
+    not, and the sustainable rate is the largest one before it."""
+    runs = []
+    for rate in rates:
+        log(f"serving sweep: {rate:g} req/s for {seconds:g} s")
+        r = serving_bench(seconds, rate, drain, seed=seed, policy=policy,
+                          errors=False)
+        r["sustainable"] = bool(
+            r["unfinished"] == 0 and r["failed"] == 0
+            and r["backlog_slope_per_s"] < 0.1 * rate)
+        runs.append(r)
+        log(f"serving sweep: {rate:g}/s finished {r['finished']} of "
+            f"{r['submitted']}, backlog slope "
+            f"{r['backlog_slope_per_s']:.2f}/s, p50 {r['p50_s']} p99 "
+            f"{r['p99_s']} s, sustainable {r['sustainable']}")
+        if not r["sustainable"]:
+            break
+    ok = [r["rate_per_s"] for r in runs if r["sustainable"]]
+    return {"window_s": seconds, "drain_s": drain, "policy": policy,
+            "sustainable_rate_per_s": max(ok) if ok else None,
+            "rule": "backlog slope over the last 2/3 of the window < 0.1 x "
+                    "rate and every request finished within the drain",
+            "runs": runs}
 
 
 def host_output_rate(eng, ntimes: int, vis_per_time: int) -> dict:
@@ -739,7 +813,13 @@ def main():
                     help="skip the grouped-HERK batch (256 mixed instances)")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving run (Poisson request stream)")
-    ap.add_argument("--serve-seconds", type=float, default=20.0,
+    ap.add_argument("--serve-sweep", default=None,
+                    help="comma-separated Poisson rates (req/s): run only "
+                         "the serving sweep at these rates and print it")
+    ap.add_argument("--serve-policy", default="fifo",
+                    choices=("fifo", "shortest"),
+                    help="admission policy of the serving sweep")
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="arrival window of the serving run")
     ap.add_argument("--serve-rate", type=float, default=20.0,
                     help="mean requests per second of the serving run")
@@ -779,6 +859,12 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.serve_sweep:
+        rates = [float(x) for x in args.serve_sweep.split(",")]
+        print(json.dumps({"serving_sweep": serving_sweep(
+            rates, args.serve_seconds, args.serve_drain,
+            policy=args.serve_policy)}))
+        return
     nant, nfreq = args.nant, args.nfreq
     log(f"building workload: {nant} antennas x {args.nsrc} sources")
     w = build_workload(nsrc=args.nsrc, nant=nant, nfreq=nfreq)

@@ -325,12 +325,17 @@ int mv_herk_reduce(int form, int nterms, int nprod, const void *z0,
  * counterpart, the reference runs one matprod per (time, frequency, chunk),
  * src/matvis/gpu/gpu.py:240-250): plain integers
  * and device addresses so that a binding can pack an array of them
- * (96 bytes each; mv_herk_problem_bytes() reports the size).  Fields as in
+ * (104 bytes each; mv_herk_problem_bytes() reports the size).  Fields as in
  * mv_herk: z = term arrays (z[2] = 0 unless 3 terms), ldz / seg_stride /
  * seg_len in complex units (seg_len the host bound), seg_len_dev = device
  * int32 column count or 0, vacc = device complex128 (ldv x ldv), scale,
  * accumulate; nslices = slice capacity given to this problem's share of the
- * workspace (mv_herk_max_slices). */
+ * workspace (mv_herk_max_slices, or more); kchain = the longest K-slice in
+ * K tiles (0: the form's cap, 192 tiles for the split forms and 64 for the
+ * plain ones; larger values are clamped to it): the problem runs
+ * ceil(K tiles / kchain) slices, at most nslices (shorter slices give a
+ * small batch more work items to spread over the GPU, at the cost of more
+ * partials to reduce); reserved = 0. */
 typedef struct mv_herk_problem {
   uint64_t z[3];
   int64_t ldz, seg_stride, seg_len;
@@ -338,6 +343,7 @@ typedef struct mv_herk_problem {
   int64_t ldv;
   double scale;
   int32_t m, nseg, accumulate, nslices;
+  int32_t kchain, reserved;
 } mv_herk_problem;
 
 size_t mv_herk_problem_bytes(void);

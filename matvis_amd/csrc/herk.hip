@@ -732,7 +732,8 @@ __global__ __launch_bounds__(256) void herk_group_plan_kernel(
     probs[p] = P;
     const int64_t seg_len = seg_columns(q.seg_len, P.seg_len_dev);
     const int ktiles = (int)(cdiv(seg_len, (int64_t)kc) * q.nseg);
-    int ns = ktiles > 0 ? icdiv(ktiles, cmax) : 1;
+    const int chain = q.kchain > 0 && q.kchain < cmax ? q.kchain : cmax;
+    int ns = ktiles > 0 ? icdiv(ktiles, chain) : 1;
     ns = ns < 1 ? 1 : (ns > q.nslices ? q.nslices : ns);
     plan.ns[p] = ns;
   }
@@ -1434,10 +1435,11 @@ extern "C" int mv_herk_grouped(int form, int nterms, int nprod, int tile,
   int64_t total_tiles = 0, max_items = 0;
   for (int i = 0; i < nprob; ++i) {
     const mv_herk_problem &q = problems_host[i];
-    MV_REQUIRE(q.m >= 1 && q.nseg >= 1 && q.seg_len >= 0 && q.nslices >= 1,
+    MV_REQUIRE(q.m >= 1 && q.nseg >= 1 && q.seg_len >= 0 &&
+                   q.nslices >= 1 && q.kchain >= 0,
                "mv_herk_grouped: problem %d: bad sizes m=%d nseg=%d "
-               "seg_len=%lld nslices=%d", i, q.m, q.nseg,
-               (long long)q.seg_len, q.nslices);
+               "seg_len=%lld nslices=%d kchain=%d", i, q.m, q.nseg,
+               (long long)q.seg_len, q.nslices, q.kchain);
     MV_REQUIRE(q.z[0] && (!split || q.z[1]) && (nterms < 3 || q.z[2]) &&
                    q.vacc,
                "mv_herk_grouped: problem %d: null pointer", i);

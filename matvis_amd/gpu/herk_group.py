@@ -35,13 +35,30 @@ def tile_for(form: int, m: int) -> int:
     return 64 if m <= 128 else 128
 
 
-#: Layout of ``mv_herk_problem`` (include/matvis_hip.h), 96 bytes.
+#: Layout of ``mv_herk_problem`` (include/matvis_hip.h), 104 bytes.
 PROBLEM_DTYPE = np.dtype([
     ("z", "<u8", (3,)), ("ldz", "<i8"), ("seg_stride", "<i8"),
     ("seg_len", "<i8"), ("seg_len_dev", "<u8"), ("vacc", "<u8"),
     ("ldv", "<i8"), ("scale", "<f8"), ("m", "<i4"), ("nseg", "<i4"),
-    ("accumulate", "<i4"), ("nslices", "<i4"),
+    ("accumulate", "<i4"), ("nslices", "<i4"), ("kchain", "<i4"),
+    ("reserved", "<i4"),
 ])
+
+#: K-tile width in complex columns of the grouped kernels, split and plain
+#: alike (kSK and kPK in csrc/herk.hip)
+K_TILE = 16
+
+
+def slice_capacity(form: int, m: int, nterms: int, nprod: int, kmax: int,
+                   kchain: int = 0) -> int:
+    """Slices a problem of up to kmax complex columns may run: the kernels'
+    own capacity (mv_herk_max_slices), raised to cover ceil(K tiles /
+    kchain) when a shorter chain is asked for."""
+    cap = Herk(form, m, nterms, nprod).nslices(kmax)
+    if kchain > 0:
+        ktiles = -(-int(kmax) // K_TILE)
+        cap = max(cap, -(-ktiles // int(kchain)))
+    return max(1, cap)
 
 
 def _addr(t) -> int:
@@ -89,9 +106,11 @@ class HerkGroup:
             host["m"][i] = p["m"]
             host["nseg"][i] = p["nseg"]
             host["accumulate"][i] = int(bool(p.get("accumulate", False)))
-            host["nslices"][i] = Herk(self.form, p["m"], self.nterms,
-                                      self.nprod).nslices(
-                p["nseg"] * p["seg_len"])
+            kchain = int(p.get("kchain", 0))
+            host["kchain"][i] = kchain
+            host["nslices"][i] = slice_capacity(
+                self.form, p["m"], self.nterms, self.nprod,
+                p["nseg"] * p["seg_len"], kchain)
         return host
 
     def update(self, problems):
@@ -100,7 +119,7 @@ class HerkGroup:
         captured graph replays with it."""
         host = self._pack(problems)
         for f in ("m", "nseg", "ldz", "seg_stride", "seg_len", "ldv",
-                  "nslices"):
+                  "nslices", "kchain"):
             if not np.array_equal(host[f], self.host[f]):
                 raise ValueError(f"grouped problems changed shape ({f})")
         self.problems, self.host = problems, host
@@ -117,20 +136,27 @@ class GroupedBatch:
     ``capture()`` records the launches in a HIP graph for ``replay()``."""
 
     def __init__(self, form: int, nterms: int, nprod: int, problems: list,
-                 tiles: list[int]):
+                 tiles: list[int], kchains: list[int] | None = None):
         self.tiles = [int(t) for t in tiles]
+        self.kchains = ([int(k) for k in kchains] if kchains is not None
+                        else None)
         classes: dict[int, list] = {}
-        for p, t in zip(problems, self.tiles):
+        for p, t in zip(self._with_chains(problems), self.tiles):
             classes.setdefault(t, []).append(p)
         self.groups = [HerkGroup(form, nterms, nprod, t, ps)
                        for t, ps in sorted(classes.items())]
         self.graph = None
 
+    def _with_chains(self, problems):
+        if self.kchains is None:
+            return problems
+        return [dict(p, kchain=k) for p, k in zip(problems, self.kchains)]
+
     def update(self, problems):
         """Same problems in the same order with new contents (see
         :meth:`HerkGroup.update`)."""
         classes: dict[int, list] = {}
-        for p, t in zip(problems, self.tiles):
+        for p, t in zip(self._with_chains(problems), self.tiles):
             classes.setdefault(t, []).append(p)
         for g in self.groups:
             g.update(classes[g.tile])

@@ -263,3 +263,64 @@ def test_grouped_accumulate_eager_and_replayed(kind):
         reset()
         HerkGroup(form, 1, 1, plain_tile(), probs).run()
         check("group")
+
+
+@pytest.mark.parametrize("kind", ["split_bf16", "c128"])
+def test_grouped_kchain_matches_single_launches(kind):
+    """Per-problem K-slice lengths (kchain): shorter slices change only the
+    FP32 (split) or FP64 (plain) partial sums.  kchain 1 on the 700-row
+    case runs 250 slices, more than mv_herk_max_slices gives (2 + 64): the
+    capacity is raised for it."""
+    from matvis_amd.gpu.herk_group import GroupedBatch, slice_capacity
+    print(f"grouped kchain {kind}", flush=True)
+    if kind == "split_bf16":
+        form, nt, npd = FORM_SPLIT_BF16, 2, 3
+        probs = [_problem(*c, form, nt, seed=40 + i)
+                 for i, c in enumerate(CASES)]
+        tiles, bound = [64 if p["m"] <= 128 else 128 for p in probs], 1e-6
+    else:
+        form, probs = _plain_problems(np.complex128, 60)
+        nt = npd = 1
+        tiles, bound = [64] * len(probs), 1e-12
+    chains = [(0, 48, 1, 7)[i % 4] for i in range(len(probs))]
+    big = CASES.index((700, 2000, 2, 1900))
+    chains[big] = 1
+    assert slice_capacity(form, 700, nt, npd, 2 * 2000, 1) == 250
+    b = GroupedBatch(form, nt, npd, probs, tiles, chains)
+    b.run()
+    for p, c, k in zip(probs, CASES, chains):
+        h, v = _single(p, form, nt, npd)
+        want = _full(v, h.ldv, p["m"], h.tile)
+        got = _full(p["vacc"], p["ldv"], p["m"], h.tile)
+        e = _nerr(got, want)
+        print(f"  case {c} kchain {k}: grouped vs single {e:.2e}", flush=True)
+        assert e < bound, (c, k, e)
+
+
+def test_autotuner_shortens_slices_of_an_underfilled_batch():
+    """Two 128-row problems of 80k columns fill few of the GPU's workgroup
+    slots at the default K-slice length (192 K tiles: 27 slices each); the
+    tuner takes shorter slices for them, is no slower than the default,
+    keeps the result within the split form's bound of a single launch, and
+    answers a second call from its cache."""
+    from matvis_amd.gpu import autotune
+    from matvis_amd.gpu.herk_group import GroupedBatch
+    print("autotuner: under-filled batch", flush=True)
+    probs = [_problem(128, 80000, 1, None, FORM_SPLIT_BF16, 2, seed=90 + i)
+             for i in range(2)]
+    plan = autotune.tune(FORM_SPLIT_BF16, 2, 3, probs, budget_s=5.0)
+    print(f"  {plan.describe()}", flush=True)
+    assert not plan.cached
+    assert all(k < 192 for k in plan.kchains), plan.describe()
+    assert plan.best_ms <= plan.default_ms
+    for p in probs:
+        p["vacc"].zero_()
+    GroupedBatch(FORM_SPLIT_BF16, 2, 3, probs, plan.tiles,
+                 plan.kchains).run()
+    for p, t in zip(probs, plan.tiles):
+        h, v = _single(p, FORM_SPLIT_BF16, 2, 3)
+        e = _nerr(_full(p["vacc"], p["ldv"], 128, t),
+                  _full(v, h.ldv, 128, h.tile))
+        assert e < 1e-6, e
+    again = autotune.tune(FORM_SPLIT_BF16, 2, 3, probs, budget_s=5.0)
+    assert again.cached and again.config == plan.config

@@ -2,8 +2,9 @@
 against references that do not share the code under test.
 
 Batched instances: 256 (time, frequency) HERK instances in grouped
-persistent launches, tiles chosen by the autotuner, captured in a HIP graph
-and replayed: nant from {64, 128, 350}, unpolarized / polarized 50/50, n_up
+persistent launches, tiles and K-slice lengths chosen by the autotuner,
+captured in a HIP graph and replayed: nant from {64, 128, 350},
+unpolarized / polarized 50/50, n_up
 log-uniform on [5k, 80k] columns per E-field axis, complex64 Z split into two
 bf16 terms (3 products).  Every instance is checked against one plain FP64
 HERK launch per instance on the same Z promoted to complex128 (the 256
@@ -88,10 +89,10 @@ def test_batched_workload_against_fp64():
 
     print("batched workload: 256 instances", flush=True)
     probs, zs = _batch()
-    tiles = autotune.choose_tiles(FORM_SPLIT_BF16, 2, 3, probs)
-    print(f"  tiles {sorted(set(tiles))}: "
-          f"{[tiles.count(t) for t in sorted(set(tiles))]}", flush=True)
-    batch = GroupedBatch(FORM_SPLIT_BF16, 2, 3, probs, tiles)
+    plan = autotune.tune(FORM_SPLIT_BF16, 2, 3, probs, budget_s=5.0)
+    print(f"  tuned: {plan.describe()}", flush=True)
+    batch = GroupedBatch(FORM_SPLIT_BF16, 2, 3, probs, plan.tiles,
+                         plan.kchains)
     # the autotuner's timing runs wrote other tile assignments into the
     # accumulators (past m too): start from zeros, as the replays do
     for p in probs:
