@@ -311,7 +311,8 @@ def test_simulate_multi_shapes_and_dtype():
 # ------------------------------------------------------------------ diffuse
 # Diffuse sky with per-antenna beams and source chunks: HEALPix pixel
 # centres, log-normal(-2, 1.5) brightness, four +-5% perturbations of one
-# E-field beam assigned round-robin (beam_idx = ant % 4), source chunks.
+# E-field beam on the 360 az x 91 za grid at 1 deg, assigned round-robin
+# (beam_idx = ant % 4), source chunks.
 
 
 def _diffuse(nside, nant, nfreq, ntimes, seed=3):
@@ -319,7 +320,7 @@ def _diffuse(nside, nant, nfreq, ntimes, seed=3):
     ra, dec = synth.healpix_centres(nside)
     flux = synth.lognormal_fluxes(ra.size, seed, -2.0, 1.5)
     antpos = synth.hex_array(nant, 14.6, 0.02, seed=seed)
-    beams = [synth.harmonic_efield_beam(seed=5, naz=72, nza=19,
+    beams = [synth.harmonic_efield_beam(seed=5, naz=360, nza=91,
                                         perturb=0.05, perturb_seed=100 + b,
                                         dtype=np.complex128)
              for b in range(4)]
@@ -456,6 +457,51 @@ def test_full_size_hermitian_and_flux_linear(full_size, method):
     # sqrt(4 I / 2) = 2 sqrt(I / 2): Z and its 16-bit split scale by 2
     # exactly, V by 4
     np.testing.assert_array_equal(out[method, 4.0], 4.0 * v)
+
+
+@pytest.mark.parametrize("freq", [100e6, 200e6])
+def test_config1_full_size_forms_match_oracle(freq):
+    """The plain-vs-mixed-precision configuration at its size: 350 hex
+    antennas (14.6 m, 2 cm jitter), 50,000 sources (RA uniform, sin(dec)
+    uniform on [-90, +30] deg, log-normal(0, 1) fluxes), polarized, the
+    360 az x 91 za complex64 E-field beam, bicubic; the first of its 30
+    times, at the ends of its 100-200 MHz band.  Each form against the
+    oracle: FP64 <= 1e-12, complex64 (precision 1) <= 1e-5, split-BF16 <=
+    1e-6 (normalised max); the per-element max is printed beside them."""
+    from matvis_amd import synth
+
+    nant, nsrc = 350, 50_000
+    antpos = synth.hex_array(nant, 14.6, 0.02, seed=1)
+    ra, dec = synth.random_sky(nsrc, 2, -90.0, 30.0)
+    flux = synth.lognormal_fluxes(nsrc, 3, 0.0, 1.0)
+    times = synth.times_spaced(1, 2.0)
+    loc = synth.hera_location()
+    rot = astro.rotation_matrices(times, loc)
+    beam = synth.harmonic_efield_beam(seed=4, naz=360, nza=91)
+    kw = dict(antpos=antpos, freqs=np.array([freq]), times=times,
+              skycoords=astro.SkyCoord(ra, dec), telescope_loc=loc,
+              fluxes=flux[:, None], beam_list=[beam], polarized=True,
+              coord_method="CoordinateRotationMatrix",
+              coord_method_params={"rotations": rot},
+              beam_spline_opts={"order": 3})
+    print(f"config 1 full size, {freq / 1e6:.0f} MHz: oracle", flush=True)
+    planes, daz, dza, azmin = O.prepare_grid(
+        beam.data_array, beam.axis1_array, beam.axis2_array, beam.freq_array,
+        freq, True, beam.beam_type)
+    ref = O.simulate(antpos=antpos, freq=freq,
+                     eci=O.point_source_crd_eq(ra, dec), rotations=rot,
+                     I_sky=flux, beam={"kind": "grid", "planes": planes[None],
+                                       "daz": [daz], "dza": [dza],
+                                       "azmin": [azmin], "order": 3},
+                     polarized=True)
+    for method, precision, bound in (("GPUMatMul", 2, 1e-12),
+                                     ("GPUMatMul", 1, 1e-5),
+                                     ("GPUMatMulSplitBF16", 2, 1e-6)):
+        v = simulate_multi(matprod_method=method, precision=precision, **kw)
+        e = nerr(v[0], ref, nant, True)
+        print(f"  {method} precision {precision}: normalised max {e[0]:.2e}"
+              f" median {e[1]:.2e}, per-element max {e[2]:.2e}", flush=True)
+        assert e[0] < bound, (method, precision, e[0])
 
 
 def test_full_size_split_within_1e6_of_fp64(full_size):
