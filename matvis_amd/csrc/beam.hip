@@ -148,6 +148,10 @@ template <> struct Taps<3> {
   }
 };
 
+// One thread per (source, plane): grid.y = nbeam x nfeed x nax planes, so a
+// polarized beam runs 4x the waves of one thread per source (at 60k
+// sources that is what keeps the tap gathers' latency covered; the planes,
+// about 1 MB for one polarized 360 x 91 complex64 beam, stay in L2).
 // AZZA: positions come as (az, za) arrays instead of topocentric vectors.
 template <typename T, typename TC, int ORDER, bool AZZA = false>
 __global__ __launch_bounds__(kThreads) void interp_kernel(
@@ -157,14 +161,15 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(
     const int32_t *__restrict__ n_dev, cplx<T> *__restrict__ out,
     int64_t ld_out, const double *__restrict__ az_in,
     const double *__restrict__ za_in) {
-  int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  const int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (p >= count) return;
+  // output plane (b, fd, ax) and its input plane (b, ax, fd)
+  const int q = blockIdx.y;
+  const int ax = q % nax, fd = (q / nax) % nfeed, b = q / (nax * nfeed);
+  cplx<T> *dst = out + (int64_t)q * ld_out + p;
   const int64_t n = n_dev ? min((int64_t)*n_dev, count) : count;
-  const int nplane = nfeed * nax;
   if (p >= n) {
-    for (int b = 0; b < nbeam; ++b)
-      for (int q = 0; q < nplane; ++q)
-        out[((int64_t)b * nplane + q) * ld_out + p] = cplx<T>{(T)0, (T)0};
+    *dst = cplx<T>{(T)0, (T)0};
     return;
   }
   double az, za;
@@ -175,62 +180,53 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(
     enu_to_az_za_uvbeam((double)crd[p], (double)crd[ld_crd + p], az, za);
   }
   const int64_t plane_sz = (int64_t)nza * naz;
-  for (int b = 0; b < nbeam; ++b) {
-    const double daz = grid[3 * b + 0], dza = grid[3 * b + 1],
-                 azmin = grid[3 * b + 2];
-    const double cx = za / dza;            // along nza
-    const double cy = (az - azmin) / daz;  // along naz
-    const bool inside =
-        cx >= 0.0 && cx <= (double)(nza - 1) && cy >= 0.0 &&
-        cy <= (double)(naz - 1);
+  const double daz = grid[3 * b + 0], dza = grid[3 * b + 1],
+               azmin = grid[3 * b + 2];
+  const double cx = za / dza;            // along nza
+  const double cy = (az - azmin) / daz;  // along naz
+  const bool inside = cx >= 0.0 && cx <= (double)(nza - 1) && cy >= 0.0 &&
+                      cy <= (double)(naz - 1);
+  double vr = 0.0, vi = 0.0;
+  if (inside) {
     double wx[Taps<ORDER>::N], wy[Taps<ORDER>::N];
     int rx[Taps<ORDER>::N], ry[Taps<ORDER>::N];
-    if (inside) {
-      int x0 = Taps<ORDER>::weights(cx, wx);
-      int y0 = Taps<ORDER>::weights(cy, wy);
+    const int x0 = Taps<ORDER>::weights(cx, wx);
+    const int y0 = Taps<ORDER>::weights(cy, wy);
 #pragma unroll
-      for (int k = 0; k < Taps<ORDER>::N; ++k) {
-        rx[k] = mirror(x0 + k, nza);
-        ry[k] = mirror(y0 + k, naz);
-      }
+    for (int k = 0; k < Taps<ORDER>::N; ++k) {
+      rx[k] = mirror(x0 + k, nza);
+      ry[k] = mirror(y0 + k, naz);
     }
-    for (int fd = 0; fd < nfeed; ++fd) {
-      for (int ax = 0; ax < nax; ++ax) {
-        double vr = 0.0, vi = 0.0;
-        if (inside) {
-          const cplx<T> *pl =
-              planes + ((int64_t)(b * nax + ax) * nfeed + fd) * plane_sz;
+    const cplx<T> *pl =
+        planes + ((int64_t)(b * nax + ax) * nfeed + fd) * plane_sz;
 #pragma unroll
-          for (int i = 0; i < Taps<ORDER>::N; ++i) {
-            double sr = 0.0, si = 0.0;
+    for (int i = 0; i < Taps<ORDER>::N; ++i) {
+      double sr = 0.0, si = 0.0;
 #pragma unroll
-            for (int j = 0; j < Taps<ORDER>::N; ++j) {
-              cplx<T> g = pl[(int64_t)rx[i] * naz + ry[j]];
-              sr += wy[j] * (double)g.re;
-              si += wy[j] * (double)g.im;
-            }
-            vr += wx[i] * sr;
-            vi += wx[i] * si;
-          }
-        }
-        cplx<T> r;
-        if (is_power) {
-          // complex sqrt of (vr, +0): the power beam's final cp.sqrt
-          if (vr >= 0.0) {
-            r.re = (T)sqrt(vr);
-            r.im = (T)0;
-          } else {
-            r.re = (T)0;
-            r.im = (T)sqrt(-vr);
-          }
-        } else {
-          r.re = (T)vr;
-          r.im = (T)vi;
-        }
-        out[((int64_t)(b * nfeed + fd) * nax + ax) * ld_out + p] = r;
+      for (int j = 0; j < Taps<ORDER>::N; ++j) {
+        const cplx<T> g = pl[(int64_t)rx[i] * naz + ry[j]];
+        sr += wy[j] * (double)g.re;
+        si += wy[j] * (double)g.im;
       }
+      vr += wx[i] * sr;
+      vi += wx[i] * si;
     }
   }
+  cplx<T> r;
+  if (is_power) {
+    // complex sqrt of (vr, +0): the power beam's final cp.sqrt
+    if (vr >= 0.0) {
+      r.re = (T)sqrt(vr);
+      r.im = (T)0;
+    } else {
+      r.re = (T)0;
+      r.im = (T)sqrt(-vr);
+    }
+  } else {
+    r.re = (T)vr;
+    r.im = (T)vi;
+  }
+  *dst = r;
 }
 
 template <typename T, typename TC>
@@ -315,8 +311,10 @@ extern "C" int mv_beam_interp(const void *planes, int is_f64, int nbeam,
   MV_REQUIRE(nbeam >= 1 && nfeed >= 1 && nax >= 1 && nza >= 1 && naz >= 1 &&
                  count >= 0 && ld_crd >= count && ld_out >= count,
              "mv_beam_interp: bad sizes");
+  MV_REQUIRE((int64_t)nbeam * nfeed * nax <= 65535,
+             "mv_beam_interp: too many planes");
   if (count == 0) return MV_OK;
-  dim3 g((unsigned)cdiv(count, kThreads));
+  dim3 g((unsigned)cdiv(count, kThreads), (unsigned)(nbeam * nfeed * nax));
   hipStream_t s = as_stream(stream);
   if (is_f64) {
     if (crd_is_f64)
@@ -353,8 +351,10 @@ extern "C" int mv_beam_interp_azza(const void *planes, int is_f64, int nbeam,
   MV_REQUIRE(nbeam >= 1 && nfeed >= 1 && nax >= 1 && nza >= 1 && naz >= 1 &&
                  count >= 0 && ld_out >= count,
              "mv_beam_interp_azza: bad sizes");
+  MV_REQUIRE((int64_t)nbeam * nfeed * nax <= 65535,
+             "mv_beam_interp_azza: too many planes");
   if (count == 0) return MV_OK;
-  dim3 g((unsigned)cdiv(count, kThreads));
+  dim3 g((unsigned)cdiv(count, kThreads), (unsigned)(nbeam * nfeed * nax));
   hipStream_t s = as_stream(stream);
 #define MV_AZZA(T, O)                                                        \
   interp_kernel<T, double, O, true><<<g, kThreads, 0, s>>>(                  \
