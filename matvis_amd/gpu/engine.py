@@ -31,6 +31,42 @@ from .matprod import f16_scale
 Z_PAD = 32  # mv_zmat_fused pads Z columns to a multiple of this
 
 
+def device_bytes(*, form: int, nterms: int, nprod: int, nant: int,
+                 polarized: bool, nfreq: int, nsrc: int, nchunks: int,
+                 chunk_size: int, nsrc_alloc: int, nbeam: int,
+                 beam_pixels: int, beam_sets: int, npairs: int,
+                 precision: int) -> dict:
+    """Device bytes an engine and its rotation plugin hold, term by term, as
+    their ``setup`` and ``run`` allocate them (beam_pixels: grid points of
+    one beam plane, 0 for analytic beams; beam_sets: 1, or nfreq when the
+    beam tables differ by frequency).  ``total`` is their sum."""
+    nf = 2 if polarized else 1
+    m = nf * nant
+    seg = max(Z_PAD, cdiv(nsrc_alloc, Z_PAD) * Z_PAD)
+    rsz = 8 if precision == 2 else 4
+    esz = 16 if form == FORM_C128 else 8  # beam values, plain Z elements
+    herk = Herk(form, m, nterms, nprod)
+    cull_ws = int(lib().mv_rotate_cull_workspace_bytes(chunk_size))
+    terms = {
+        # rotation plugin: ECI, topocentric (and FP64 / BCRS copies), the
+        # kept coordinates, fluxes, indices and its cull workspace
+        "coords": nsrc * (24 + 3 * rsz + 24 + 24 + 16 + 8)
+        + nsrc_alloc * (3 * rsz + 4 * 16 + 4) + cull_ws,
+        "antpos_flux": nant * 24 + nfreq * nsrc * 8,
+        "chunk_coords": nchunks * seg * (24 + 4) + cull_ws,
+        "beam_tables": beam_sets * nbeam * nf * nf * beam_pixels * esz,
+        "beam_values": (nchunks if beam_sets == 1 else 1) * nbeam * nf * nf
+        * seg * esz,
+        "z": m * nf * seg * (4 * nterms if is_split(form) else esz),
+        "vacc": herk.ldv * herk.ldv * 16,
+        "herk_workspace": int(lib().mv_herk_workspace_bytes(
+            form, m, herk.nslices(nf * nsrc_alloc))),
+        "output": 2 * nfreq * npairs * nf * nf * 2 * rsz,
+    }
+    terms["total"] = sum(terms.values())
+    return terms
+
+
 class VisibilityEngine:
     """Device-resident state of one simulation and its per-time pipeline.
 

@@ -90,9 +90,15 @@ def build_engine(
         freemem, min_chunks, inner_beams, nax, nfeed, nant, nsrc, precision,
         source_buffer=source_buffer, memory_buffer=memory_buffer,
     )
-    nsrc_alloc = int(npixc * source_buffer)
     beams, _, beam_idx = _wrangle_beams(beam_idx, beam_list, polarized, nant,
                                         float(freqs[0]))
+    nchunks, npixc = fit_chunks(
+        freemem * memory_buffer, nchunks, form=form, nterms=nterms,
+        nprod=nprod, nant=nant, polarized=polarized, nfreq=freqs.size,
+        nsrc=nsrc, source_buffer=source_buffer, beams=beams,
+        npairs=nant * nant if antpairs is None else len(antpairs),
+        precision=precision)
+    nsrc_alloc = int(npixc * source_buffer)
     coord_cls = CoordinateRotation._methods[coord_method]
     coords = coord_cls(
         flux=np.sqrt(0.5 * fluxes[:, 0]), times=times,
@@ -112,6 +118,49 @@ def build_engine(
     logger.info(f"Running with {nchunks} chunks, form {form}")
     engine.setup()
     return engine, ntimes
+
+
+def fit_chunks(budget: float, nchunks: int, *, form, nterms, nprod, nant,
+               polarized, nfreq, nsrc, source_buffer, beams, npairs,
+               precision) -> tuple[int, int]:
+    """(nchunks, sources per chunk): the reference's chunk count
+    (get_desired_chunks, which sizes the reference's own buffers), raised
+    until this build's device footprint (engine.device_bytes: Z terms,
+    split-K workspace, FP64 accumulator, beam tables and values, output
+    buffers, the rotation plugin's arrays) fits the budget
+    (min(max_memory, free memory) x memory_buffer).  If even min(nsrc, 1000)
+    chunks do not fit, that count is used and a warning logged."""
+    from ..beams import is_grid_beam, unwrap
+    from .engine import device_bytes
+
+    grid = [unwrap(b)[0] for b in beams if is_grid_beam(b)]
+    pixels = max((int(np.prod(g.data_array.shape[-2:])) + 3
+                  * int(g.data_array.shape[-2]) for g in grid), default=0)
+    per_freq = nfreq > 1 and any(np.asarray(g.freq_array).size > 1
+                                 for g in grid)
+
+    def need(ch):
+        npc = int(np.ceil(nsrc / ch))
+        return device_bytes(
+            form=form, nterms=nterms, nprod=nprod, nant=nant,
+            polarized=polarized, nfreq=nfreq, nsrc=nsrc, nchunks=ch,
+            chunk_size=npc, nsrc_alloc=int(npc * source_buffer),
+            nbeam=len(beams), beam_pixels=pixels,
+            beam_sets=nfreq if per_freq else 1, npairs=npairs,
+            precision=precision)["total"]
+
+    limit = max(nchunks, min(nsrc, 1000))
+    ch = nchunks
+    while ch < limit and need(ch) > budget:
+        ch = min(limit, ch + max(1, ch // 8))
+    if need(ch) > budget:
+        logger.warning(
+            f"device footprint {need(ch) / 2**30:.2f} GiB with {ch} chunks "
+            f"exceeds the budget of {budget / 2**30:.2f} GiB")
+    elif ch != nchunks:
+        logger.info(f"{ch} chunks (not {nchunks}) to fit this build's "
+                    f"footprint in {budget / 2**30:.2f} GiB")
+    return ch, int(np.ceil(nsrc / ch))
 
 
 def simulate_multi(**kwargs):

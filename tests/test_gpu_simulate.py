@@ -546,3 +546,57 @@ def test_registered_rotations_are_repeatable():
         assert not np.array_equal(got[0], got[1]), name
         checked += 1
     assert checked >= 3
+
+
+def test_max_memory_bounds_device_allocation():
+    """max_memory is a bound on what this build allocates: FP64 at 350
+    antennas x 100k sources under 1.5 GiB runs in 3 chunks (the reference's
+    chunk model alone would give 2, which do not fit the split-K partials),
+    the peak of the device allocations stays within the budget and within
+    the footprint model, which overestimates it by at most 25%, and the
+    visibilities equal the single-chunk run's within 1e-12."""
+    import torch
+
+    from matvis_amd import synth
+    from matvis_amd.gpu.engine import device_bytes
+    from matvis_amd.gpu.gpu import build_engine
+
+    nant, nsrc, budget = 350, 100_000, int(1.5 * 2**30)
+    antpos = synth.hex_array(nant, 14.6, 0.02, seed=1)
+    ra, dec = synth.random_sky(nsrc, 2, -90.0, 30.0)
+    flux = synth.lognormal_fluxes(nsrc, 3, 0.0, 1.0)
+    times = synth.times_spaced(1, 2.0)
+    loc = synth.hera_location()
+    beam = synth.harmonic_efield_beam(seed=4, naz=360, nza=91)
+    kw = dict(antpos=antpos, freqs=np.array([150e6]), times=times,
+              skycoords=astro.SkyCoord(ra, dec), telescope_loc=loc,
+              fluxes=flux[:, None], beam_list=[beam], polarized=True,
+              precision=2, coord_method="CoordinateRotationMatrix",
+              coord_method_params={"rotations": astro.rotation_matrices(
+                  times, loc)},
+              beam_spline_opts={"order": 3})
+    print("max_memory 1.5 GiB, FP64 350 x 100k", flush=True)
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    base = torch.cuda.memory_allocated()
+    torch.cuda.reset_peak_memory_stats()
+    eng, nt = build_engine(max_memory=budget, **kw)
+    v = eng.run(nt)
+    torch.cuda.synchronize()
+    peak = torch.cuda.max_memory_allocated() - base
+    model = device_bytes(
+        form=eng.form, nterms=eng.nterms, nprod=eng.nprod, nant=nant,
+        polarized=True, nfreq=1, nsrc=nsrc, nchunks=eng.nchunks,
+        chunk_size=eng.chunk_size, nsrc_alloc=eng.nsrc_alloc, nbeam=1,
+        beam_pixels=91 * 363, beam_sets=1, npairs=nant * nant,
+        precision=2)["total"]
+    print(f"  {eng.nchunks} chunks; peak {peak / 2**30:.3f} GiB, model "
+          f"{model / 2**30:.3f} GiB, budget {budget / 2**30:.2f} GiB",
+          flush=True)
+    assert eng.nchunks == 3
+    assert peak <= budget
+    assert peak <= model <= 1.25 * peak
+    del eng
+    torch.cuda.empty_cache()
+    one = simulate_multi(**kw)
+    assert nerr(v[0], one[0], nant, True)[0] < 1e-12

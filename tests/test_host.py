@@ -285,3 +285,44 @@ def test_data_path_resolution(tmp_path, monkeypatch):
     got = matvis_amd._data_path()
     assert got.name == "data" and got.parent.name in ("matvis_amd", "matvis")
 
+
+
+def test_chunks_fit_this_builds_footprint():
+    """max_memory bounds this build's device memory: the reference's chunk
+    model (which sizes its own buffers) is the starting point, raised while
+    the footprint model (Z terms, split-K partials, accumulator, beam
+    tables and values, outputs, the rotation plugin's arrays) exceeds the
+    budget.  FP64 at 350 antennas x 100k sources under 1.5 GiB: the
+    reference's model gives 2 chunks, the partials need a third."""
+    from matvis_amd._lib import FORM_C128, FORM_SPLIT_BF16
+    from matvis_amd._utils import get_desired_chunks
+    from matvis_amd.gpu.engine import device_bytes
+    from matvis_amd.gpu.gpu import fit_chunks
+
+    beam = synth.harmonic_efield_beam(seed=4, naz=360, nza=91)
+    budget = 1.5 * 2**30
+    kw = dict(nant=350, polarized=True, nfreq=1, nsrc=100_000,
+              source_buffer=1.0, beams=[beam], npairs=350 * 350,
+              precision=2)
+    ref, _ = get_desired_chunks(budget, 1, [beam], 2, 2, 350, 100_000, 2)
+    assert ref == 2
+    ch, npc = fit_chunks(0.9 * budget, ref, form=FORM_C128, nterms=1,
+                         nprod=1, **kw)
+    assert (ch, npc) == (3, 33334)
+
+    def total(form, nt, npd, nch):
+        n = -(-100_000 // nch)
+        return device_bytes(form=form, nterms=nt, nprod=npd, nant=350,
+                            polarized=True, nfreq=1, nsrc=100_000,
+                            nchunks=nch, chunk_size=n, nsrc_alloc=n,
+                            nbeam=1, beam_pixels=91 * 363, beam_sets=1,
+                            npairs=350 * 350, precision=2)["total"]
+
+    assert total(FORM_C128, 1, 1, 2) > 0.9 * budget >= total(FORM_C128, 1, 1,
+                                                              3)
+    # split-BF16 fits at the reference's count: unchanged
+    assert fit_chunks(0.9 * budget, ref, form=FORM_SPLIT_BF16, nterms=2,
+                      nprod=3, **kw)[0] == ref
+    # a budget nothing fits: the cap, not an endless loop
+    assert fit_chunks(1e6, 1, form=FORM_C128, nterms=1, nprod=1,
+                      **kw)[0] == 1000
