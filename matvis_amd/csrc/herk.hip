@@ -819,58 +819,74 @@ __global__ __launch_bounds__(256) void herk_plain_grouped_kernel(
 // FP64 (fixed order), scaled, stored or added into the accumulator.  With
 // `dpart` (split forms) the diagonal elements of diagonal tiles take the row
 // norms summed there instead of the MFMA sums.
-template <typename TP>
-__device__ __forceinline__ void reduce_pair(
+//
+// A thread takes kReduceEpt adjacent elements of one row, as eight
+// independent sums over slices s = 8q + j combined in a fixed order, so
+// eight loads are in flight.  (Four elements a thread halve the grid of the
+// FP32-partial reduce to 1.3 workgroups a CU and measured slower.)
+template <typename TP> constexpr int kReduceEpt = 2;
+
+template <typename TP, bool DTILE>
+__device__ __forceinline__ void reduce_elems_t(
     const cplx<TP> *__restrict__ part, const double *__restrict__ dpart,
-    int ntiles, int nb, int ts, int tile, int64_t e, int ns, double scale,
-    cplx<double> *__restrict__ vacc, int64_t ldv, int accumulate) {
-  // elements e and e + 1 (e even) of one tile: the same row, adjacent
-  // columns; eight independent sums over slices s = 8q + j per element,
-  // combined in a fixed order, so eight 16-byte loads are in flight
+    int ntiles, int bi, int bj, int ts, int tile, int64_t e, int ns,
+    double scale, cplx<double> *__restrict__ vacc, int64_t ldv,
+    int accumulate) {
+  constexpr int E = kReduceEpt<TP>;
   const int64_t tsz = (int64_t)ts * ts;
-  int bi, bj;
-  tile_coords(tile, nb, bi, bj);
+  const int64_t r = e / ts, c = e % ts;
   const cplx<TP> *src = part + (int64_t)tile * tsz + e;
   const int64_t step = (int64_t)ntiles * tsz;
-  double re[2][8], im[2][8];
+  // diagonal tiles (DTILE) also sum the row norms of the thread's row, in
+  // the same loop (every lane of the tile, so no lane runs extra rounds of
+  // dependent loads after it; a wave's lanes share two rows' addresses)
+  constexpr bool dtile = DTILE;
+  const double *dp = dtile ? dpart + (int64_t)tile * ts + r : nullptr;
+  const int64_t dstep = (int64_t)ntiles * ts;
+  double re[E][8], im[E][8], dn[8];
 #pragma unroll
-  for (int k = 0; k < 2; ++k)
+  for (int j = 0; j < 8; ++j) {
+    dn[j] = 0.0;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) re[k][j] = im[k][j] = 0.0;
+    for (int k = 0; k < E; ++k) re[k][j] = im[k][j] = 0.0;
+  }
   int s = 0;
   for (; s + 8 <= ns; s += 8) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const cplx<TP> *q = src + (int64_t)(s + j) * step;
-      const cplx<TP> a = q[0], b = q[1];
-      re[0][j] += (double)a.re;
-      im[0][j] += (double)a.im;
-      re[1][j] += (double)b.re;
-      im[1][j] += (double)b.im;
+      cplx<TP> v[E];
+#pragma unroll
+      for (int k = 0; k < E; ++k) v[k] = q[k];
+      if constexpr (dtile) dn[j] += dp[(int64_t)(s + j) * dstep];
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        re[k][j] += (double)v[k].re;
+        im[k][j] += (double)v[k].im;
+      }
     }
   }
   for (; s < ns; ++s) {
     const cplx<TP> *q = src + (int64_t)s * step;
-    const cplx<TP> a = q[0], b = q[1];
-    re[0][0] += (double)a.re;
-    im[0][0] += (double)a.im;
-    re[1][0] += (double)b.re;
-    im[1][0] += (double)b.im;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      re[k][0] += (double)q[k].re;
+      im[k][0] += (double)q[k].im;
+    }
+    if constexpr (dtile) dn[0] += dp[(int64_t)s * dstep];
   }
-  const int64_t r = e / ts, c = e % ts;
   cplx<double> *dst = vacc + ((int64_t)bi * ts + r) * ldv + (int64_t)bj * ts + c;
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < E; ++k) {
     double sr = ((re[k][0] + re[k][1]) + (re[k][2] + re[k][3])) +
                 ((re[k][4] + re[k][5]) + (re[k][6] + re[k][7]));
     double si = ((im[k][0] + im[k][1]) + (im[k][2] + im[k][3])) +
                 ((im[k][4] + im[k][5]) + (im[k][6] + im[k][7]));
-    if (dpart && bi == bj && r == c + k) {
+    if (dtile && r == c + k) {
       // the diagonal: the FP64 sum of |terms|^2 the compute kernel kept
-      sr = 0.0;
+      sr = ((dn[0] + dn[1]) + (dn[2] + dn[3])) +
+           ((dn[4] + dn[5]) + (dn[6] + dn[7]));
       si = 0.0;
-      for (int t = 0; t < ns; ++t)
-        sr += dpart[((int64_t)t * ntiles + tile) * ts + r];
     }
     if (accumulate) {
       dst[k].re += sr * scale;
@@ -881,8 +897,29 @@ __device__ __forceinline__ void reduce_pair(
   }
 }
 
-// grid (ts^2 / 512, ntiles): one thread per two elements of each upper
-// tile; the slice count is found once per block
+template <typename TP>
+__device__ __forceinline__ void reduce_elems(
+    const cplx<TP> *__restrict__ part, const double *__restrict__ dpart,
+    int ntiles, int nb, int ts, int tile, int64_t e, int ns, double scale,
+    cplx<double> *__restrict__ vacc, int64_t ldv, int accumulate) {
+  int bi, bj;
+  tile_coords(tile, nb, bi, bj);
+  if (dpart && bi == bj)  // uniform over the block: one tile a block
+    reduce_elems_t<TP, true>(part, dpart, ntiles, bi, bj, ts, tile, e, ns,
+                             scale, vacc, ldv, accumulate);
+  else
+    reduce_elems_t<TP, false>(part, dpart, ntiles, bi, bj, ts, tile, e, ns,
+                              scale, vacc, ldv, accumulate);
+}
+
+// Blocks of 256 threads, kReduceEpt<TP> elements a thread, per tile.
+template <typename TP> dim3 reduce_grid(int ts, int64_t ntiles) {
+  return dim3((unsigned)cdiv((int64_t)ts * ts,
+                             (int64_t)256 * kReduceEpt<TP>),
+              (unsigned)ntiles);
+}
+
+// grid reduce_grid<TP>(ts, ntiles); the slice count is found once per block
 template <typename TP>
 __global__ __launch_bounds__(256) void herk_reduce_kernel(
     const cplx<TP> *__restrict__ part, const double *__restrict__ dpart,
@@ -898,28 +935,30 @@ __global__ __launch_bounds__(256) void herk_reduce_kernel(
     if (threadIdx.x == 0) ns_shared = ns;
   }
   __syncthreads();
-  const int64_t e = 2 * ((int64_t)blockIdx.x * 256 + threadIdx.x);
+  const int64_t e =
+      kReduceEpt<TP> * ((int64_t)blockIdx.x * 256 + threadIdx.x);
   if (e >= (int64_t)ts * ts) return;
-  reduce_pair(part, dpart, ntiles, nb, ts, (int)blockIdx.y, e, ns_shared,
-              scale, vacc, ldv, accumulate);
+  reduce_elems(part, dpart, ntiles, nb, ts, (int)blockIdx.y, e, ns_shared,
+               scale, vacc, ldv, accumulate);
 }
 
 // Grouped reduce: blockIdx.y runs over the tiles of all problems (prefix
-// offsets from the plan); two elements per thread as above.
+// offsets from the plan); elements per thread as above.
 template <typename TP>
 __global__ __launch_bounds__(256) void herk_group_reduce_kernel(
     const SplitProblem *__restrict__ probs, int nprob, int ts,
     GroupPlan plan) {
-  const int64_t e = 2 * ((int64_t)blockIdx.x * 256 + threadIdx.x);
+  const int64_t e =
+      kReduceEpt<TP> * ((int64_t)blockIdx.x * 256 + threadIdx.x);
   if (e >= (int64_t)ts * ts) return;
   // the grid covers the tiles of the problems' largest shapes (a graph
   // replayed with smaller ones keeps it); tiles past this plan's total idle
   if ((int32_t)blockIdx.y >= plan.tile_off[nprob]) return;
   const int p = find_segment(plan.tile_off, nprob, (int32_t)blockIdx.y);
   const SplitProblem &P = probs[p];
-  reduce_pair((const cplx<TP> *)P.part, P.dpart, P.ntiles, P.nb, ts,
-              (int)blockIdx.y - plan.tile_off[p], e, plan.ns[p], P.scale,
-              P.vacc, P.ldv, P.accumulate);
+  reduce_elems((const cplx<TP> *)P.part, P.dpart, P.ntiles, P.nb, ts,
+               (int)blockIdx.y - plan.tile_off[p], e, plan.ns[p], P.scale,
+               P.vacc, P.ldv, P.accumulate);
 }
 
 __global__ __launch_bounds__(256) void vacc_sum_kernel(
@@ -1170,8 +1209,6 @@ int herk_run(int phases, int form, int nterms, int nprod, const void *z0,
     return MV_ERR_HIP;
   }
   hipStream_t st = as_stream(stream);
-  const dim3 rgrid((unsigned)cdiv((int64_t)ts * ts / 2, (int64_t)256),
-                   (unsigned)ntiles);
   if (form == MV_FORM_C128) {
     if (phases & kPhaseCompute) {
       herk_plain_kernel<double><<<p.grid, 256, 0, st>>>(
@@ -1180,7 +1217,8 @@ int herk_run(int phases, int form, int nterms, int nprod, const void *z0,
       if (int e = check_launch("mv_herk/plain64")) return e;
     }
     if (!(phases & kPhaseReduce)) return MV_OK;
-    herk_reduce_kernel<double><<<rgrid, 256, 0, st>>>(
+    herk_reduce_kernel<double>
+        <<<reduce_grid<double>(ts, ntiles), 256, 0, st>>>(
         (const cplx<double> *)workspace, nullptr, p.ntiles, p.nb, ts, seg_len,
         seg_len_dev, nseg, p.kc, p.cfg, scale, (cplx<double> *)vacc, ldv,
         accumulate);
@@ -1194,7 +1232,8 @@ int herk_run(int phases, int form, int nterms, int nprod, const void *z0,
       if (int e = check_launch("mv_herk/plain32")) return e;
     }
     if (!(phases & kPhaseReduce)) return MV_OK;
-    herk_reduce_kernel<float><<<rgrid, 256, 0, st>>>(
+    herk_reduce_kernel<float>
+        <<<reduce_grid<float>(ts, ntiles), 256, 0, st>>>(
         (const cplx<float> *)workspace, nullptr, p.ntiles, p.nb, ts, seg_len,
         seg_len_dev, nseg, p.kc, p.cfg, scale, (cplx<double> *)vacc, ldv,
         accumulate);
@@ -1249,7 +1288,7 @@ int herk_run(int phases, int form, int nterms, int nprod, const void *z0,
   }
 #undef MV_SPLIT_LAUNCH
   if (!(phases & kPhaseReduce)) return MV_OK;
-  herk_reduce_kernel<float><<<rgrid, 256, 0, st>>>(
+  herk_reduce_kernel<float><<<reduce_grid<float>(ts, ntiles), 256, 0, st>>>(
       (const cplx<float> *)workspace, dpart, p.ntiles, p.nb, ts, seg_len,
       seg_len_dev, nseg, p.kc, p.cfg, scale, (cplx<double> *)vacc, ldv,
       accumulate);
@@ -1501,21 +1540,21 @@ extern "C" int mv_herk_grouped(int form, int nterms, int nprod, int tile,
       ws + g.parts, plan);
   if (int e = check_launch("mv_herk_grouped/plan")) return e;
   const int grid = (int)(max_items < slots ? max_items : slots);
-  const dim3 rgrid((unsigned)cdiv((int64_t)tile * tile / 2, (int64_t)256),
-                   (unsigned)total_tiles);
   if (!split) {
     if (form == MV_FORM_C128) {
       herk_plain_grouped_kernel<double><<<grid, 256, 0, st>>>(probs, nprob,
                                                               plan);
       if (int e = check_launch("mv_herk_grouped/plain")) return e;
-      herk_group_reduce_kernel<double><<<rgrid, 256, 0, st>>>(probs, nprob,
-                                                             tile, plan);
+      herk_group_reduce_kernel<double>
+          <<<reduce_grid<double>(tile, total_tiles), 256, 0, st>>>(
+              probs, nprob, tile, plan);
     } else {
       herk_plain_grouped_kernel<float><<<grid, 256, 0, st>>>(probs, nprob,
                                                              plan);
       if (int e = check_launch("mv_herk_grouped/plain")) return e;
-      herk_group_reduce_kernel<float><<<rgrid, 256, 0, st>>>(probs, nprob,
-                                                            tile, plan);
+      herk_group_reduce_kernel<float>
+          <<<reduce_grid<float>(tile, total_tiles), 256, 0, st>>>(
+              probs, nprob, tile, plan);
     }
     return check_launch("mv_herk_grouped/reduce");
   }
@@ -1541,8 +1580,9 @@ extern "C" int mv_herk_grouped(int form, int nterms, int nprod, int tile,
 #undef MV_GROUP_TILE
 #undef MV_GROUP_LAUNCH
   if (int e = check_launch("mv_herk_grouped/split")) return e;
-  herk_group_reduce_kernel<float><<<rgrid, 256, 0, st>>>(probs, nprob, tile,
-                                                        plan);
+  herk_group_reduce_kernel<float>
+      <<<reduce_grid<float>(tile, total_tiles), 256, 0, st>>>(probs, nprob,
+                                                             tile, plan);
   return check_launch("mv_herk_grouped/reduce");
 }
 
