@@ -654,6 +654,7 @@ STAGE_KERNELS = [
     ("HERK (split-BF16)", "mv::herk_split_kernel<2, 1, false, 128>",
      "nfreq"),
     ("split-K reduce", "mv::herk_reduce_kernel<float>", "nfreq"),
+    ("expansion to the API layout", "mv::expand_kernel<double>", "nfreq"),
 ]
 
 
