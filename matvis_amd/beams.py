@@ -67,8 +67,10 @@ class GaussianBeam:
     E-field amplitude ``exp(-za^2 / (2 sigma^2))`` (power = its square), with
     ``sigma`` in radians scaled by ``(freq / reference_frequency) **
     spectral_index`` when both are set.  ``diameter`` (m) sets sigma from an
-    Airy-disk match, ``arcsin(2.2150894 * lambda / (pi D)) * 2 / 2.355``
-    (unpinned: restates pyuvdata's GaussianBeam, which is absent here).
+    Airy-disk match, ``arcsin(2.2150894 * lambda / (pi D)) * 2 / 2.355``:
+    the Gaussian E-field's half maximum where the Airy E-field 2 J1(x) / x
+    has its own (x = 2.2150894).  Restates pyuvdata's GaussianBeam, which is
+    absent here.
     """
 
     sigma: float | None = None

@@ -44,10 +44,13 @@ Parity status per function (see ``tests/test_oracle.py``):
                             default kind (its documentation; absent here)
 ``interp_grid_beam``        pinned: scipy map_coordinates; node identity and
                             linear exactness (test_beam_interp_gpu.py:12-64)
-``gaussian_beam``           **parity unpinned**: restates pyuvdata's
-                            GaussianBeam e-field ``exp(-za^2 / (2 sigma^2))``
-                            (and ``/sqrt(2)`` per feed/axis when polarized);
-                            pyuvdata is absent, nothing checks the formula
+``gaussian_beam``           pinned in part: with the diameter rule's sigma
+                            the E-field ``exp(-za^2 / (2 sigma^2))`` is 1/2
+                            at the Airy E-field's half maximum (scipy j1;
+                            a power-beam sigma or the intensity half
+                            maximum fails); **parity unpinned**: the
+                            ``/sqrt(2)`` per feed/axis when polarized and
+                            pyuvdata's 2.355 (pyuvdata is absent)
 ``exptau``                  pinned: zenith source gives exactly 1; closed form
                             ``exp(+2 pi i nu b.s / c)`` on a 1-antenna case
 ``zmatrix``                 pinned: per-antenna beam loop == broadcast
@@ -390,7 +393,8 @@ def gaussian_beam(sigma, za, polarized: bool):
     ``exp(-za^2 / (2 sigma^2))``; unpolarized simulations take the square
     root of the xx power beam, which is that amplitude; polarized ones see
     the amplitude divided by sqrt(2) on every (feed, axis).
-    PARITY UNPINNED: pyuvdata is absent, nothing checks this formula.
+    Pinned in part (the E-field convention against the Airy half maximum of
+    the diameter rule); the polarized normalization is parity unpinned.
     """
     sigma = np.atleast_1d(np.asarray(sigma, dtype=np.float64))
     za = np.asarray(za, dtype=np.float64)

@@ -420,3 +420,31 @@ def test_prepare_grid_power_before_frequency_step():
     np.testing.assert_allclose(planes[0, 0], want, rtol=1e-13)
     other = (np.abs(at(nu)[:, 0]) ** 2).sum(0)
     assert np.abs(other - want).max() > 1e-6 * np.abs(want).max()
+
+
+def test_gaussian_beam_matches_airy_half_maximum():
+    """The diameter rule of GaussianBeam (sigma = arcsin(2.2150894 lambda /
+    (pi D)) x 2 / 2.355) matches the Gaussian's E-field half maximum to the
+    Airy pattern's: 2.2150894 is where the Airy E-field 2 J1(x) / x falls to
+    1/2, and the oracle's E-field exp(-za^2 / (2 sigma^2)) is 1/2 at that
+    angle (to the 1e-4 of 2.355 against 2 sqrt(2 ln 2)).  A sigma taken as
+    the power beam's, the intensity half maximum (x = 1.6163) or a missing
+    factor 2 fails."""
+    from scipy.optimize import brentq
+    from scipy.special import j1
+
+    from matvis_amd.beams import GaussianBeam
+
+    x_half = brentq(lambda x: 2.0 * j1(x) / x - 0.5, 1.0, 3.0, xtol=1e-14)
+    assert abs(x_half - 2.2150894) < 1e-6
+    for D, f in ((14.0, 150e6), (8.0, 100e6), (14.0, 200e6)):
+        lam = 299792458.0 / f
+        theta = np.arcsin(x_half * lam / (np.pi * D))
+        sigma = GaussianBeam(diameter=D).get_sigma(f)
+        e = O.gaussian_beam(sigma, np.array([theta]), False)[0, 0, 0, 0].real
+        assert abs(e - 0.5) < 2e-4, (D, f, e)
+        # the plausible mistakes
+        assert abs(np.exp(-theta**2 / sigma**2) - 0.5) > 0.1
+        s_int = np.arcsin(1.6163 * lam / (np.pi * D)) * 2 / 2.355
+        assert abs(np.exp(-theta**2 / (2 * s_int**2)) - 0.5) > 0.05
+        assert abs(np.exp(-theta**2 / (2 * (sigma / 2) ** 2)) - 0.5) > 0.1
