@@ -16,15 +16,17 @@ R_t feeds the device kernels and the host oracle, so parity of the per-time
 computation does not depend on this accuracy.
 
 :func:`erfa_context` gives the per-time astrometry context of the ERFA-style
-rotation (the fields of ERFA's ``eraApco13`` that matvis's ERFA path uses):
-the observer's heliocentric direction and distance and barycentric velocity
-from the low-precision solar series of the Astronomical Almanac
-(differentiated for the velocity; the Sun's motion about the barycentre and
-the Moon's pull on the Earth are omitted, ~1e-3 of the velocity) plus the
-observer's rotational velocity, the Earth rotation angle (IAU 2000, UT1 =
-UTC) and a bias-precession-nutation matrix built from the model above so
-that the context reproduces R_t exactly when light deflection and
-aberration are left out.
+rotation (the fields of ERFA's ``eraApco13`` that matvis's ERFA path uses).
+From 2015 to 2045 the date-dependent inputs -- the CIP coordinates x, y and
+CIO locator s of the IAU 2006/2000A bias-precession-nutation, the Earth's
+heliocentric position and barycentric velocity -- are interpolated (cubic,
+daily nodes) from ``data/astrometry_2015_2045.npz``, written with ERFA by
+``data/make_astrometry_table.py``; the observer's geocentric position and
+velocity (WGS84, IAU 2000 Earth rotation angle, UT1 = UTC, no polar motion)
+and the GCRS -> CIRS matrix from (x, y, s) are formed here.  Outside that
+range the context falls back to the low-precision solar series of the
+Astronomical Almanac and the R_t model above (0.05-0.09 arcsec from ERFA),
+marked ``approx``.
 """
 
 from __future__ import annotations
@@ -35,14 +37,16 @@ import numpy as np
 
 ARCSEC = math.pi / (180.0 * 3600.0)
 
-#: measured distance of the built-in context's apparent directions from
-#: ERFA's ``eraAtco13`` (arcsec; 48 sources x 5 dates at the HERA site)
+#: measured distance of the low-precision series context's apparent
+#: directions from ERFA's ``eraAtco13`` (arcsec; 48 sources x 5 dates at the
+#: HERA site); the table-based context (2015-2045) is within 0.0031-0.0044
 BUILTIN_CONTEXT_ERROR_ARCSEC = (0.05, 0.09)
 
 
 class AstrometryAccuracyWarning(UserWarning):
-    """The built-in astrometry model is in use: apparent directions are
-    0.05-0.09 arcsec from ERFA's, not within the reference's 10 mas."""
+    """The low-precision series context is in use (a date outside the
+    astrometry table's 2015-2045): apparent directions are 0.05-0.09 arcsec
+    from ERFA's, not within the reference's 10 mas."""
 J2000 = 2451545.0
 TT_MINUS_UTC_S = 69.184  # TAI-UTC (37 s) + 32.184 s, valid from 2017
 
@@ -281,14 +285,122 @@ def sun_geocentric(jd_tt: float):
     return r * u, dr * u + r * dlam * du
 
 
+# ------------------------------------------------------------ ERFA-table path
+
+_TABLE = None
+_TABLE_PATH = None
+
+
+def _table():
+    """The daily astrometry table (loaded once), or None if absent."""
+    global _TABLE, _TABLE_PATH
+    if _TABLE is None:
+        from pathlib import Path
+
+        path = Path(__file__).with_name("data") / "astrometry_2015_2045.npz"
+        if not path.exists():
+            _TABLE = {}
+            return None
+        with np.load(path) as d:
+            _TABLE = {k: np.asarray(d[k]) for k in ("x", "y", "s", "eh",
+                                                     "vb")}
+            _TABLE["jd0"] = float(d["jd_utc0"])
+        _TABLE_PATH = path
+    return _TABLE or None
+
+
+def _interp_daily(values: np.ndarray, jd0: float, jd: float):
+    """4-point Lagrange interpolation of daily samples at jd (needs a node
+    on each side of the two around jd); None outside the table."""
+    t = jd - jd0
+    i = int(math.floor(t))
+    if i < 1 or i + 2 >= len(values):
+        return None
+    u = t - i  # in [0, 1): nodes i - 1, i, i + 1, i + 2 at -1, 0, 1, 2
+    w = np.array([-u * (u - 1.0) * (u - 2.0) / 6.0,
+                  (u + 1.0) * (u - 1.0) * (u - 2.0) / 2.0,
+                  -(u + 1.0) * u * (u - 2.0) / 2.0,
+                  (u + 1.0) * u * (u - 1.0) / 6.0])
+    return np.tensordot(w, values[i - 1:i + 3], axes=(0, 0))
+
+
+def c2i_from_xys(x: float, y: float, s: float) -> np.ndarray:
+    """GCRS -> CIRS matrix from the CIP coordinates and the CIO locator
+    (IERS Conventions 2010, eq. 5.10: R3(-(E + s)) R2(d) R3(E))."""
+    r2 = x * x + y * y
+    e = math.atan2(y, x) if r2 > 0.0 else 0.0
+    d = math.atan(math.sqrt(r2 / (1.0 - r2)))
+    return _r3(-(e + s)) @ _r2(d) @ _r3(e)
+
+
+def observer_geocentric(lat: float, lon: float, height: float,
+                        era: float):
+    """Position (m) and velocity (m/s) of a WGS84 site in the frame that the
+    Earth rotation angle era rotates (no polar motion)."""
+    e2 = WGS84_F * (2.0 - WGS84_F)
+    nrad = WGS84_A / math.sqrt(1.0 - e2 * math.sin(lat) ** 2)
+    rxy = (nrad + height) * math.cos(lat)
+    z = (nrad * (1.0 - e2) + height) * math.sin(lat)
+    x0, y0 = rxy * math.cos(lon), rxy * math.sin(lon)
+    c, s = math.cos(era), math.sin(era)
+    pos = np.array([c * x0 - s * y0, s * x0 + c * y0, z])
+    om = ERA_RATE / 86400.0
+    vel = om * np.array([-s * x0 - c * y0, c * x0 - s * y0, 0.0])
+    return pos, vel
+
+
+AU_M = 149597870700.0
+
+
+def _table_context(jd_utc, lat, lon, height):
+    tab = _table()
+    if tab is None:
+        return None
+    vals = [_interp_daily(tab[k], tab["jd0"], jd_utc)
+            for k in ("x", "y", "s", "eh", "vb")]
+    if any(v is None for v in vals):
+        return None
+    x, y, s_cio, eh_earth, vb_earth = vals
+    bpn = c2i_from_xys(float(x), float(y), float(s_cio))
+    era = earth_rotation_angle(jd_utc)
+    pos, vel = observer_geocentric(lat, lon, height, era)
+    # observer in GCRS (bpn takes GCRS to CIRS; the ERA frame is CIRS)
+    pos_g, vel_g = bpn.T @ pos, bpn.T @ vel
+    ph = eh_earth + pos_g / AU_M
+    em = float(np.linalg.norm(ph))
+    v = (vb_earth + vel_g * 86400.0 / AU_M) / C_AU_PER_DAY
+    return {
+        "eh": ph / em, "em": em, "v": v, "bm1": math.sqrt(1.0 - float(v @ v)),
+        "bpn": bpn, "eral": (era + lon) % (2.0 * math.pi), "xpl": 0.0,
+        "ypl": 0.0, "sphi": math.sin(lat), "cphi": math.cos(lat),
+        "approx": False,
+    }
+
+
 def erfa_context(jd_utc: float, lat: float, lon: float,
                  height: float = 0.0) -> dict:
     """Astrometry context at one UTC Julian date for a geodetic site
     (radians, metres): ``eh`` (unit vector Sun -> observer, GCRS), ``em``
     (au), ``v`` (observer barycentric velocity / c, GCRS), ``bm1``,
-    ``bpn`` (GCRS -> the frame ``eral`` rotates), ``eral`` (local Earth
-    rotation angle), ``xpl = ypl = 0`` (no polar motion), ``sphi``,
-    ``cphi``.  UT1 = UTC."""
+    ``bpn`` (GCRS -> CIRS), ``eral`` (local Earth rotation angle), ``xpl =
+    ypl = 0`` (no polar motion), ``sphi``, ``cphi``, and ``approx`` (True
+    outside the table's 2015-2045, where the low-precision series gives the
+    context).  UT1 = UTC."""
+    ctx = _table_context(jd_utc, lat, lon, height)
+    if ctx is not None:
+        return ctx
+    ctx = erfa_context_series(jd_utc, lat, lon, height)
+    ctx["approx"] = True
+    return ctx
+
+
+def erfa_context_series(jd_utc: float, lat: float, lon: float,
+                        height: float = 0.0) -> dict:
+    """The context of :func:`erfa_context` from the low-precision series
+    (Almanac solar series, IAU 1976 precession, four IAU 1980 nutation
+    terms): 0.05-0.09 arcsec from ERFA, at any date; its ``bpn`` takes GCRS
+    to the frame ``eral`` rotates, so that with light deflection and
+    aberration left out it reproduces R_t."""
     jd_tt = jd_utc + TT_MINUS_UTC_S / 86400.0
     prec = precession_iau1976(jd_tt)
     dpsi, deps, eps0 = nutation_iau1980_leading(jd_tt)

@@ -15,9 +15,10 @@ context comes from, in order: ``astrom`` (caller-supplied, one dict per
 time with the fields of ERFA's ``eraApco13``: eh, em, v, bm1, bpn, eral,
 xpl, ypl, sphi, cphi), astropy's ``erfa_astrom`` when the times are astropy
 ``Time`` objects (what the reference calls, cpu/coords.py:165-166), else the
-host model :func:`matvis_amd.astro.erfa_context` (0.05-0.09 arcsec from
-ERFA's full chain against the reference's 10 mas; UT1 = UTC, no polar
-motion), which warns once per rotation object with
+host model :func:`matvis_amd.astro.erfa_context` (UT1 = UTC, no polar
+motion): within 3-5 mas of ERFA's full chain from 2015 to 2045, where it
+interpolates ERFA-generated daily tables, and 0.05-0.09 arcsec outside,
+where it warns once per rotation object with
 :class:`matvis_amd.astro.AstrometryAccuracyWarning`.
 """
 
@@ -105,20 +106,21 @@ class CoordinateRotationERFA(CoordinateRotation):
         ctx = _astropy_context(self.times[t], self.telescope_loc)
         if ctx is not None:
             return ctx
-        if not self._warned_builtin:
-            lo, hi = BUILTIN_CONTEXT_ERROR_ARCSEC
-            warnings.warn(
-                "no astrometry context from astropy or the caller: using the "
-                f"built-in model, {1e3 * lo:.0f}-{1e3 * hi:.0f} mas from "
-                "ERFA's eraAtco13 (the reference's ERFA path is within 10 "
-                "mas); pass astropy Time objects or astrom= contexts for "
-                "full accuracy", AstrometryAccuracyWarning, stacklevel=2)
-            self._warned_builtin = True
         lat, lon = location_lat_lon(self.telescope_loc)
         height = getattr(self.telescope_loc, "height", 0.0)
         height = float(height.to_value("m") if hasattr(height, "to_value")
                        else height)
-        return erfa_context(float(self._jd[t]), lat, lon, height)
+        ctx = erfa_context(float(self._jd[t]), lat, lon, height)
+        if ctx.get("approx") and not self._warned_builtin:
+            lo, hi = BUILTIN_CONTEXT_ERROR_ARCSEC
+            warnings.warn(
+                "a date outside the astrometry table (2015-2045): using the "
+                f"low-precision series, {1e3 * lo:.0f}-{1e3 * hi:.0f} mas "
+                "from ERFA's eraAtco13 (the reference's ERFA path is within "
+                "10 mas); pass astropy Time objects or astrom= contexts for "
+                "full accuracy", AstrometryAccuracyWarning, stacklevel=2)
+            self._warned_builtin = True
+        return ctx
 
     def rotate(self, t: int) -> None:
         """BCRS corrections when due (cpu/coords.py:186-215), then the

@@ -64,11 +64,35 @@ def test_oracle_chain_planted_errors_fail():
 
 
 @pytest.mark.parametrize("i", range(NT))
-def test_host_context_close_to_erfa(i):
+def test_host_context_matches_erfa(i):
+    """The host context from the ERFA-generated daily tables (2015-2045)
+    against eraApco13's own at the golden dates (two of them a fraction of
+    a day past a node): the full chain within the reference's 10 mas of
+    eraAtco13 (tests/test_coordrot.py:124-125)."""
     from matvis_amd import astro
 
     lon, lat, hm = G["site"]
     c = astro.erfa_context(float(G["utc"][i]), lat, lon, hm)
+    g = ctx(i)
+    assert not c["approx"]
+    d_eral = (c["eral"] - g["eral"] + np.pi) % (2 * np.pi) - np.pi
+    assert abs(d_eral) < 1e-9
+    np.testing.assert_allclose(c["bpn"], g["bpn"], rtol=0, atol=2e-9)
+    np.testing.assert_allclose(c["eh"], g["eh"], rtol=0, atol=5e-9)
+    np.testing.assert_allclose(c["v"], g["v"], rtol=0, atol=1e-10)
+    assert abs(c["em"] - g["em"]) < 5e-9
+    topo = O.erfa_atioq_matrix(c) @ O.erfa_bcrs(eci(), c)
+    assert ang(topo, G["obs_enu"][i]).max() < 10 * MAS
+
+
+@pytest.mark.parametrize("i", range(NT))
+def test_series_context_close_to_erfa(i):
+    """The low-precision series context (dates outside the table): within
+    150 mas of eraAtco13."""
+    from matvis_amd import astro
+
+    lon, lat, hm = G["site"]
+    c = astro.erfa_context_series(float(G["utc"][i]), lat, lon, hm)
     g = ctx(i)
     d_eral = (c["eral"] - g["eral"] + np.pi) % (2 * np.pi) - np.pi
     assert abs(d_eral) < 1e-9
@@ -78,15 +102,32 @@ def test_host_context_close_to_erfa(i):
     assert ang(topo, G["obs_enu"][i]).max() < 150 * MAS
 
 
-def test_host_context_without_corrections_is_the_rotation_matrix():
-    """With light deflection and aberration left out the context reproduces
-    R_t of astro.eci_to_enu_rotation."""
+def test_table_interpolation_and_range():
+    """The daily table: nodes interpolate to themselves, the cubic is exact
+    for cubic data, and dates outside 2015-2045 fall back to the series
+    (marked approx)."""
+    from matvis_amd import astro
+
+    assert astro._table() is not None
+    vals = np.arange(10.0) ** 3 - 2 * np.arange(10.0)
+    for jd in (3.0, 4.25, 6.999):
+        got = astro._interp_daily(vals, 0.0, jd)
+        assert abs(got - (jd ** 3 - 2 * jd)) < 1e-9
+    assert astro._interp_daily(vals, 0.0, 0.5) is None
+    assert astro._interp_daily(vals, 0.0, 8.0) is None
+    lon, lat, hm = G["site"]
+    assert astro.erfa_context(2469000.5, lat, lon, hm)["approx"]  # 2048
+
+
+def test_series_context_without_corrections_is_the_rotation_matrix():
+    """With light deflection and aberration left out the series context
+    reproduces R_t of astro.eci_to_enu_rotation."""
     from matvis_amd import astro
     from matvis_amd.gpu.coords import atioq_matrix
 
     lon, lat, hm = G["site"]
     for jd in G["utc"]:
-        c = astro.erfa_context(float(jd), lat, lon, hm)
+        c = astro.erfa_context_series(float(jd), lat, lon, hm)
         np.testing.assert_allclose(atioq_matrix(c) @ c["bpn"],
                                    astro.eci_to_enu_rotation(jd, lat, lon),
                                    rtol=0, atol=1e-12)
@@ -246,8 +287,9 @@ def test_gpu_erfa_precision1_and_chunk_sizes():
 
 
 def test_builtin_context_warns_once():
-    """The built-in host model is 0.05-0.09 arcsec from ERFA: using it warns
-    (once per rotation object); caller-supplied contexts do not."""
+    """The series context (outside 2015-2045) is 0.05-0.09 arcsec from ERFA:
+    using it warns (once per rotation object); the table context and
+    caller-supplied contexts do not."""
     import warnings
 
     from matvis_amd import astro
@@ -256,16 +298,20 @@ def test_builtin_context_warns_once():
     lon, lat, hm = G["site"]
     loc = astro.EarthLocation.from_geodetic(np.degrees(lon), np.degrees(lat),
                                             hm)
-    kw = dict(flux=np.ones(G["ra"].size), times=G["utc"], telescope_loc=loc,
+    late = np.array([2469000.5, 2469000.6])  # 2048, past the table
+    kw = dict(flux=np.ones(G["ra"].size), telescope_loc=loc,
               skycoords=astro.SkyCoord(G["ra"], G["dec"]), precision=2)
-    co = CoordinateRotationERFA(0.0, **kw)
+    co = CoordinateRotationERFA(0.0, times=late, **kw)
     with pytest.warns(astro.AstrometryAccuracyWarning, match="mas"):
         c = co.context(0)
+    assert c["approx"]
     np.testing.assert_array_equal(
-        c["bpn"], astro.erfa_context(float(G["utc"][0]), lat, lon,
-                                     hm)["bpn"])
+        c["bpn"], astro.erfa_context_series(float(late[0]), lat, lon,
+                                            hm)["bpn"])
     with warnings.catch_warnings():
         warnings.simplefilter("error")
         co.context(1)  # once per object
+        CoordinateRotationERFA(0.0, times=G["utc"], **kw).context(0)
         CoordinateRotationERFA(
-            0.0, astrom=[ctx(i) for i in range(NT)], **kw).context(0)
+            0.0, times=G["utc"], astrom=[ctx(i) for i in range(NT)],
+            **kw).context(0)
