@@ -12,6 +12,9 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "csrc" / "libmatvis_hip.so"
 
+#: the library's mv_abi_version() this binding is written for
+ABI_VERSION = 2
+
 MV_OK = 0
 MV_ERR_ARG = -1
 MV_ERR_HIP = -2
@@ -109,6 +112,12 @@ def lib() -> C.CDLL:
             fn = getattr(handle, name)
             fn.argtypes = argtypes
             fn.restype = _RESTYPES.get(name, _I)
+        got = handle.mv_abi_version()
+        if got != ABI_VERSION:
+            raise ImportError(
+                f"{LIB_PATH} has ABI version {got}, this binding needs "
+                f"{ABI_VERSION}: rebuild it with `python -m "
+                "matvis_amd.csrc.build`")
         _lib = handle
     return _lib
 
