@@ -1,0 +1,114 @@
+"""Edge cases of the drop-in entry point, in every matprod form, against the
+CPU oracle: a sky with no source above the horizon at any time (the result
+is exactly zero, tests/test_matvis_gpu.py:15-25 generalised), one antenna
+(m = 1 or 2, a single diagonal tile that is mostly padding), and a single
+source (K = 2 columns, one K tile that is mostly zero padding).
+
+Bounds (normalised max error, |V - V_ref| / sqrt(|V_ref_ii| |V_ref_jj|)):
+FP64 1e-12; complex64 1e-5; two-term bf16 split 3e-5 (with one source
+nothing averages the 2^-17 per-component representation residual of the
+two terms down; the diagonal is exact); fp16 split 2e-6.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from _problems import small_problem
+from matvis_amd import astro
+from matvis_amd.gpu.gpu import simulate
+
+pytestmark = pytest.mark.gpu
+
+FORMS = {
+    "fp64": (dict(precision=2, matprod_method="GPUMatMul"), 1e-12),
+    "c64": (dict(precision=1, matprod_method="GPUMatMul"), 1e-5),
+    "vectordot": (dict(precision=2, matprod_method="GPUVectorDot"), 1e-12),
+    "bf16": (dict(precision=2, matprod_method="GPUMatMulSplitBF16"), 3e-5),
+    "fp16": (dict(precision=2, matprod_method="GPUMatMulSplitF16"), 2e-6),
+}
+
+
+def _run(p, **kw):
+    opts = dict(antpos=p["antpos"], freq=p["freq"], times=p["times"],
+                skycoords=astro.SkyCoord(p["ra"], p["dec"]),
+                telescope_loc=p["loc"], I_sky=p["I_sky"],
+                beam_list=p["beam_list"], polarized=p["polarized"],
+                coord_method="CoordinateRotationMatrix",
+                coord_method_params={"rotations": p["rotations"]},
+                beam_spline_opts={"order": p["order"]})
+    opts.update(kw)
+    return simulate(**opts)
+
+
+def _oracle(p):
+    return O.simulate(antpos=p["antpos"], freq=p["freq"], eci=p["eci"],
+                      rotations=p["rotations"], I_sky=p["I_sky"],
+                      beam=p["oracle_beam"], polarized=p["polarized"])
+
+
+def _with_sky(p, ra, dec, flux):
+    p = dict(p)
+    p["ra"], p["dec"] = np.asarray(ra, float), np.asarray(dec, float)
+    p["eci"] = O.point_source_crd_eq(p["ra"], p["dec"])
+    p["I_sky"] = np.asarray(flux, float)
+    return p
+
+
+def _nerr(v, ref, nant, polarized):
+    nf = 2 if polarized else 1
+    if not polarized:
+        v, ref = v[..., None, None], ref[..., None, None]
+    return O.normalized_error(v, ref, nant, nf)
+
+
+@pytest.mark.parametrize("form", list(FORMS))
+def test_no_source_above_the_horizon_is_exact_zero(form):
+    """Declinations above 90 deg + latitude never rise at latitude -30.72
+    deg: every chunk is empty at every time, in every form."""
+    kw, _ = FORMS[form]
+    p = small_problem(nant=5, nsrc=50, ntimes=3, polarized=True, beam="grid",
+                      order=3)
+    rng = np.random.default_rng(3)
+    p = _with_sky(p, rng.uniform(0, 2 * np.pi, 50),
+                  np.radians(rng.uniform(62.0, 89.0, 50)),
+                  rng.uniform(0.5, 2.0, 50))
+    v = _run(p, **kw)
+    assert v.shape == (3, 25, 2, 2)
+    assert np.all(v == 0)
+    assert np.all(_oracle(p) == 0)
+
+
+@pytest.mark.parametrize("polarized", [True, False])
+@pytest.mark.parametrize("form", list(FORMS))
+def test_one_antenna_matches_oracle(form, polarized):
+    kw, bound = FORMS[form]
+    p = small_problem(nant=1, nsrc=600, ntimes=2, polarized=polarized,
+                      beam="grid", order=3)
+    v = _run(p, **kw)
+    ref = _oracle(p)
+    assert v.shape == ref.shape
+    e = _nerr(v, ref, 1, polarized)
+    print(f"one antenna {form} pol={polarized}: normalised max {e[0]:.2e}",
+          flush=True)
+    assert e[0] < bound
+
+
+@pytest.mark.parametrize("form", list(FORMS))
+def test_one_source_matches_oracle(form):
+    """One source near the zenith: V_ij = z_i conj(z_j) exactly, one K tile
+    of two live columns among zero padding."""
+    kw, bound = FORMS[form]
+    p = small_problem(nant=9, nsrc=1, ntimes=2, polarized=True, beam="grid",
+                      order=1)
+    # the zenith at the first time: the rotation's third row is the local
+    # vertical in ECI coordinates
+    up = p["rotations"][0][2]
+    p = _with_sky(p, [np.arctan2(up[1], up[0]) + 0.05],
+                  [np.arcsin(up[2]) - 0.03], [3.0])
+    v = _run(p, **kw)
+    ref = _oracle(p)
+    assert np.any(ref != 0)
+    e = _nerr(v, ref, 9, True)
+    print(f"one source {form}: normalised max {e[0]:.2e}", flush=True)
+    assert e[0] < bound
