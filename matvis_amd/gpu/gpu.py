@@ -21,6 +21,7 @@ from . import coords as _coords  # noqa: F401  (registers rotation methods)
 from . import matprod as mp
 from .beams import spline_order
 from .engine import VisibilityEngine
+from .herk import products_for
 
 logger = logging.getLogger(__name__)
 
@@ -36,8 +37,8 @@ def _resolve_matprod(matprod_method: str, precision: int, split_terms,
 
         form = FORM_C128 if precision == 2 else FORM_C64
         return form, 1, 1
-    return (form, split_terms or cls.default_terms,
-            split_products or cls.default_products)
+    nterms = split_terms or cls.default_terms
+    return form, nterms, products_for(nterms, split_products)
 
 
 def build_engine(

@@ -25,6 +25,15 @@ FORMS = {
 K_TILE = 16
 #: (nterms, nprod) pairs the split kernels implement.
 SPLIT_VARIANTS = {(2, 3), (2, 4), (3, 6)}
+#: products taken when only the term count is given: 2 terms drop lo*lo,
+#: 3 terms drop the three products below FP32 resolution (m*l, l*m, l*l)
+DEFAULT_PRODUCTS = {2: 3, 3: 6}
+
+
+def products_for(nterms: int, nprod: int | None) -> int:
+    """The product count of a split form: as given, else the default for
+    the term count."""
+    return nprod or DEFAULT_PRODUCTS.get(nterms, 1)
 
 
 def cdiv(a: int, b: int) -> int:

@@ -20,7 +20,7 @@ import torch
 from .. import _device as D
 from .._lib import FORM_C64, FORM_C128, FORM_SPLIT_BF16, FORM_SPLIT_F16, call
 from ..core.matprod import MatProd
-from .herk import Herk, expand
+from .herk import Herk, expand, products_for
 
 
 class GPUMatMul(MatProd):
@@ -36,7 +36,9 @@ class GPUMatMul(MatProd):
                  split_products: int | None = None):
         super().__init__(nchunks, nfeed, nant, antpairs, precision)
         self.nterms = split_terms or self.default_terms
-        self.nprod = split_products or self.default_products
+        self.nprod = (products_for(self.nterms, split_products)
+                      if self.split_form is not None
+                      else split_products or self.default_products)
         if self.split_form is None:
             self.form = FORM_C128 if precision == 2 else FORM_C64
         else:

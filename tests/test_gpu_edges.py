@@ -7,7 +7,9 @@ source (K = 2 columns, one K tile that is mostly zero padding).
 Bounds (normalised max error, |V - V_ref| / sqrt(|V_ref_ii| |V_ref_jj|)):
 FP64 1e-12; complex64 1e-5; two-term bf16 split 3e-5 (with one source
 nothing averages the 2^-17 per-component representation residual of the
-two terms down; the diagonal is exact); fp16 split 2e-6.
+two terms down; the diagonal is exact); fp16 split and three bf16 terms
+2e-6; three bf16 terms at precision 1 (complex64 Z, complex64 output)
+1e-5, the complex64 bound.
 """
 
 import numpy as np
@@ -26,6 +28,11 @@ FORMS = {
     "vectordot": (dict(precision=2, matprod_method="GPUVectorDot"), 1e-12),
     "bf16": (dict(precision=2, matprod_method="GPUMatMulSplitBF16"), 3e-5),
     "fp16": (dict(precision=2, matprod_method="GPUMatMulSplitF16"), 2e-6),
+    # three bf16 terms (six products) from the term count alone
+    "bf16x3": (dict(precision=2, matprod_method="GPUMatMulSplitBF16",
+                    split_terms=3), 2e-6),
+    "bf16x3_p1": (dict(precision=1, matprod_method="GPUMatMulSplitBF16",
+                       split_terms=3), 1e-5),
 }
 
 

@@ -165,6 +165,26 @@ def test_single_precision_matches_oracle():
     assert e[0] < 1e-5
 
 
+def test_three_term_split_at_precision_1_betters_complex64():
+    """Three bf16 terms hold a complex64 Z exactly (3 x 8 significand bits);
+    the six kept products drop only terms below FP32 resolution, and the
+    diagonal is summed exactly.  At precision 1 the form stays within the
+    complex64 bound and is no less accurate than the plain complex64 HERK on
+    the same problem (normalised max and median)."""
+    p = small_problem(nant=20, nsrc=4000, ntimes=2, polarized=True,
+                      beam="grid", order=3)
+    ref = run_oracle(p)
+    v64 = run_gpu(p, precision=1)
+    v3 = run_gpu(p, precision=1, matprod_method="GPUMatMulSplitBF16",
+                 split_terms=3)
+    assert v3.dtype == np.complex64
+    e64, e3 = nerr(v64, ref, 20, True), nerr(v3, ref, 20, True)
+    print(f"precision 1: complex64 {e64[0]:.2e} (median {e64[1]:.2e}), "
+          f"3-term bf16 {e3[0]:.2e} (median {e3[1]:.2e})", flush=True)
+    assert e3[0] < 1e-5
+    assert e3[0] <= 1.1 * e64[0] and e3[1] <= 1.1 * e64[1]
+
+
 # Bounds on this 20-antenna, 4000-source bicubic problem: the two-term bf16
 # split leaves a 2^-18 representation residual per element, about 1e-6 here
 # with 3 or 4 products (the diagonal is summed exactly, so lo*lo only

@@ -239,6 +239,28 @@ def test_matprod_antpairs_validation():
     assert mp.npairs == 9 and mp.antpairs[4].tolist() == [1, 1]
 
 
+def test_split_products_default_follows_terms():
+    """Given only the term count, a split form takes the products that its
+    kernels implement for it: 2 terms -> 3, 3 terms -> 6; plain forms 1."""
+    from matvis_amd._lib import FORM_C64, FORM_SPLIT_BF16, FORM_SPLIT_F16
+    from matvis_amd.gpu.gpu import _resolve_matprod
+    from matvis_amd.gpu.herk import SPLIT_VARIANTS
+    from matvis_amd.gpu.matprod import GPUMatMul, GPUMatMulSplitBF16
+
+    r = _resolve_matprod
+    assert r("GPUMatMulSplitBF16", 2, None, None) == (FORM_SPLIT_BF16, 2, 3)
+    assert r("GPUMatMulSplitBF16", 2, 3, None) == (FORM_SPLIT_BF16, 3, 6)
+    assert r("GPUMatMulSplitBF16", 2, 2, 4) == (FORM_SPLIT_BF16, 2, 4)
+    assert r("GPUMatMulSplitF16", 1, 3, None) == (FORM_SPLIT_F16, 3, 6)
+    assert r("GPUMatMul", 1, None, None) == (FORM_C64, 1, 1)
+    for t in (2, 3):
+        assert (t, r("GPUMatMulSplitBF16", 2, t, None)[2]) in SPLIT_VARIANTS
+    mp = GPUMatMulSplitBF16(1, 2, 3, None, precision=2, split_terms=3)
+    assert (mp.nterms, mp.nprod) == (3, 6)
+    assert (GPUMatMul(1, 2, 3, None).nterms, GPUMatMul(1, 2, 3, None).nprod) \
+        == (1, 1)
+
+
 @pytest.mark.parametrize("nf", [1, 2])
 def test_product_error_metric_matches_oracle(nf):
     """The metric the bench reports equals the oracle's definition, and a
