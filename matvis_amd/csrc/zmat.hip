@@ -12,6 +12,9 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kZPad = 32;  // Z columns are zero-padded to a multiple of this
+// longest Z or term row (elements): a lane's column within a row is a 32-bit
+// offset from the row start
+constexpr int64_t kMaxRow = int64_t(1) << 30;
 
 __device__ __forceinline__ double reduced_turns(const double *__restrict__ ap,
                                                 double fc, double sx,
@@ -82,81 +85,114 @@ __global__ __launch_bounds__(kThreads) void zmat_parts_kernel(
 
 // ---------------------------------------------------------------- splitting
 
-__device__ __forceinline__ uint32_t bf16_bits_rn(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return u >> 16;
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+// x and y rounded to the nearest-even 16-bit value, x in the low half of the
+// word and y in the high half (one v_cvt_pk_bf16_f32 for bf16)
+template <bool F16>
+__device__ __forceinline__ uint32_t pack16(float x, float y) {
+  if constexpr (F16)
+    return __builtin_bit_cast(uint32_t,
+                              __builtin_convertvector((f32x2){x, y}, f16x2));
+  else
+    return __builtin_bit_cast(uint32_t,
+                              __builtin_convertvector((f32x2){x, y}, bf16x2));
 }
-__device__ __forceinline__ float bf16_value(uint32_t h) {
-  return __uint_as_float(h << 16);
+// the values of the low and high halves of a packed word
+template <bool F16>
+__device__ __forceinline__ float half_lo(uint32_t w) {
+  if constexpr (F16)
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xFFFFu));
+  else
+    return __uint_as_float(w << 16);
 }
-__device__ __forceinline__ uint32_t f16_bits_rn(float x) {
-  _Float16 h = (_Float16)x;
-  return (uint32_t)__builtin_bit_cast(uint16_t, h);
-}
-__device__ __forceinline__ float f16_value(uint32_t h) {
-  return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
+template <bool F16>
+__device__ __forceinline__ float half_hi(uint32_t w) {
+  if constexpr (F16)
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+  else
+    return __uint_as_float(w & 0xFFFF0000u);
 }
 
-// Splits x into NT 16-bit terms x ~ t0 + t1 + ..., each the round-to-nearest
-// of the remainder the previous terms left (exact in FP32).
+// Splits (re, im) into NT 16-bit term pairs, re ~ sum of the low halves and
+// im ~ sum of the high halves of t[0..NT): each term the round-to-nearest of
+// the remainder the previous terms left (the remainders are exact in FP32).
 template <bool F16, int NT>
-__device__ __forceinline__ void split_terms(float x, uint32_t *t) {
-  float r = x;
+__device__ __forceinline__ void split_terms(float re, float im, uint32_t *t) {
 #pragma unroll
   for (int k = 0; k < NT; ++k) {
-    uint32_t b = F16 ? f16_bits_rn(r) : bf16_bits_rn(r);
-    t[k] = b;
-    r = r - (F16 ? f16_value(b) : bf16_value(b));
+    t[k] = pack16<F16>(re, im);
+    if (k + 1 < NT) {
+      re -= half_lo<F16>(t[k]);
+      im -= half_hi<F16>(t[k]);
+    }
   }
 }
 
+// Writers of one Z element at (row offset, column): `row` is the offset of
+// the row (and segment) start, the same in every lane of the wave; `col` the
+// lane's column within it, below 2^30 (the entry points check the row length).
 template <int FORM, int NT> struct ZWriter;
 template <> struct ZWriter<MV_FORM_C128, 1> {
   typedef double T;
-  __device__ static void put(void *const *zs, int64_t off, cplx<double> v) {
-    ((cplx<double> *)zs[0])[off] = v;
+  __device__ static void put(void *const *zs, int64_t row, uint32_t col,
+                             cplx<double> v) {
+    ((cplx<double> *)zs[0] + row)[col] = v;
   }
 };
 template <> struct ZWriter<MV_FORM_C64, 1> {
   typedef float T;
-  __device__ static void put(void *const *zs, int64_t off, cplx<float> v) {
-    ((cplx<float> *)zs[0])[off] = v;
+  __device__ static void put(void *const *zs, int64_t row, uint32_t col,
+                             cplx<float> v) {
+    ((cplx<float> *)zs[0] + row)[col] = v;
   }
 };
 // Split forms: per term, 16-byte chunks of 4 consecutive columns, the real
 // parts of the 4 columns (16 bits each) in the chunk's first 8 bytes and
 // their imaginary parts in its last 8.  Column c of a row is slot c % 4 of
 // chunk c / 4 (rows and segments start on chunk boundaries).  Lanes 4q ..
-// 4q + 3 hold the 4 columns of one chunk; each lane trades a half-word with
-// its xor-1 partner and stores one whole word: even slots the word of real
-// parts, odd slots the word of imaginary parts.  Every lane of a quad must
-// call put() (the exchange is a wave operation).
+// 4q + 3 hold the 4 columns of one chunk.  Each lane packs its (re, im) term
+// into one word, takes its xor-1 partner's word (a DPP quad permutation
+// within the quad) and stores one whole word: even slots j the real parts
+// of columns j, j + 1 (word j / 2), odd slots the imaginary parts of
+// columns j - 1, j (word 2 + j / 2), formed by one byte permutation of the
+// two words.  Every lane of a quad must call put() together.
 template <bool F16, int NT>
-__device__ __forceinline__ void put_split(void *const *zs, int64_t off,
-                                          cplx<float> v) {
-  uint32_t r[NT], i[NT];
-  split_terms<F16, NT>(v.re, r);
-  split_terms<F16, NT>(v.im, i);
-  const int j = (int)(off & 3);
-  const bool odd = j & 1;
-  const int64_t w = off - j + (odd ? 2 : 0) + (j >> 1);
+__device__ __forceinline__ void put_split(void *const *zs, int64_t row,
+                                          uint32_t col, cplx<float> v) {
+  uint32_t t[NT];
+  split_terms<F16, NT>(v.re, v.im, t);
+  const uint32_t j = col & 3u;
+  const bool odd = j & 1u;
+  // byte offset of the lane's word within the row: 32 bits, so that the
+  // store takes the row start as a scalar base
+  const uint32_t wb = (col - j + (odd ? 2u : 0u) + (j >> 1)) * 4u;
+  // bytes of {partner, mine}: even lanes take my low half then the
+  // partner's low half; odd lanes the partner's high half then my high half
+  const uint32_t sel = odd ? 0x03020706u : 0x05040100u;
 #pragma unroll
   for (int k = 0; k < NT; ++k) {
-    const uint32_t recv = (uint32_t)__shfl_xor((int)(odd ? r[k] : i[k]), 1, 64);
-    ((uint32_t *)zs[k])[w] = odd ? (recv | (i[k] << 16)) : (r[k] | (recv << 16));
+    // quad_perm [1, 0, 3, 2]: the word of lane l ^ 1
+    const uint32_t p =
+        (uint32_t)__builtin_amdgcn_mov_dpp((int)t[k], 0xB1, 0xF, 0xF, false);
+    char *base = (char *)((uint32_t *)zs[k] + row);
+    *(uint32_t *)(base + wb) = __builtin_amdgcn_perm(p, t[k], sel);
   }
 }
 template <int NT> struct ZWriter<MV_FORM_SPLIT_BF16, NT> {
   typedef float T;
-  __device__ static void put(void *const *zs, int64_t off, cplx<float> v) {
-    put_split<false, NT>(zs, off, v);
+  __device__ static void put(void *const *zs, int64_t row, uint32_t col,
+                             cplx<float> v) {
+    put_split<false, NT>(zs, row, col, v);
   }
 };
 template <int NT> struct ZWriter<MV_FORM_SPLIT_F16, NT> {
   typedef float T;
-  __device__ static void put(void *const *zs, int64_t off, cplx<float> v) {
-    put_split<true, NT>(zs, off, v);
+  __device__ static void put(void *const *zs, int64_t row, uint32_t col,
+                             cplx<float> v) {
+    put_split<true, NT>(zs, row, col, v);
   }
 };
 
@@ -224,9 +260,9 @@ __global__ __launch_bounds__(kThreads) void zmat_fused_kernel(
     for (int fd = 0; fd < nf; ++fd)
 #pragma unroll
       for (int ax = 0; ax < na; ++ax)
-        ZWriter<FORM, NT>::put(
-            zo.z, ((int64_t)a * nf + fd) * ldz + (int64_t)ax * seg_stride + s,
-            out[fd * na + ax]);
+        ZWriter<FORM, NT>::put(zo.z, ((int64_t)a * nf + fd) * ldz,
+                               (uint32_t)(ax * seg_stride + s),
+                               out[fd * na + ax]);
   }
 }
 
@@ -245,7 +281,7 @@ __global__ __launch_bounds__(kThreads) void split_kernel(
     const cplx<TI> v = z[r * ldz + sg * seg_stride + c];
     w = cplx<float>{(float)v.re * scale, (float)v.im * scale};
   }
-  ZWriter<FORM, NT>::put(to.z, r * ldt + sg * tseg_stride + c, w);
+  ZWriter<FORM, NT>::put(to.z, r * ldt + sg * tseg_stride, (uint32_t)c, w);
 }
 
 template <int FORM, int NT>
@@ -350,6 +386,7 @@ extern "C" int mv_zmat_fused(const double *antpos, int nant, double freq,
              (long long)seg_stride, kZPad);
   MV_REQUIRE(ldz >= (int64_t)nax * seg_stride,
              "mv_zmat_fused: ldz must be >= nax * seg_stride");
+  MV_REQUIRE(ldz <= kMaxRow, "mv_zmat_fused: ldz must be at most 2^30");
   MV_REQUIRE((form != MV_FORM_SPLIT_BF16 && form != MV_FORM_SPLIT_F16) ||
                  (ldz % 4 == 0 && seg_stride % 4 == 0),
              "mv_zmat_fused: split terms need ldz and seg_stride multiples "
@@ -408,6 +445,7 @@ extern "C" int mv_split_z(const void *z, int z_is_f64, int64_t rows, int nseg,
                  seg_stride >= cols && ldz >= nseg * seg_stride &&
                  tseg_stride >= cols && ldt >= nseg * tseg_stride,
              "mv_split_z: bad sizes");
+  MV_REQUIRE(ldt <= kMaxRow, "mv_split_z: ldt must be at most 2^30");
   MV_REQUIRE(ldt % 4 == 0 && (nseg == 1 || tseg_stride % 4 == 0),
              "mv_split_z: term rows and segments must start on 4-column "
              "chunks (ldt and tseg_stride multiples of 4)");

@@ -7,6 +7,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cassert>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -132,6 +133,37 @@ inline emu_u32x2 emu_permlane32_swap(unsigned x, unsigned y) {
   return r;
 }
 #define __builtin_amdgcn_permlane32_swap(x, y, fi, bc) emu_permlane32_swap(x, y)
+// v_mov_b32_dpp with a quad_perm control (ctrl < 0x100, the only DPP form
+// the sources use): lane l takes the value of lane 4 (l / 4) + sel_l of its
+// quad, sel_l = bits 2 (l % 4) .. 2 (l % 4) + 1 of ctrl.
+inline int emu_mov_dpp(int v, int ctrl) {
+  assert(ctrl >= 0 && ctrl < 0x100);
+  const int l = emu_lane();
+  memcpy(emu_wave_slot(l), &v, sizeof v);
+  emu_wave_barrier();
+  const int src = (l & ~3) | ((ctrl >> (2 * (l & 3))) & 3);
+  int r = v;
+  if (emu_lane_live(src)) memcpy(&r, emu_wave_slot(src), sizeof r);
+  emu_wave_barrier();
+  return r;
+}
+#define __builtin_amdgcn_mov_dpp(v, ctrl, rm, bm, bc) emu_mov_dpp(v, ctrl)
+// v_perm_b32: byte i of the result is byte sel_i of the 8 bytes {s0, s1}
+// (s1 in bytes 0-3, s0 in 4-7); selector 12 gives 0 and 13-255 give 0xff
+// (8-11, the sign replications, are not used by the sources)
+inline unsigned emu_perm(unsigned s0, unsigned s1, unsigned sel) {
+  const unsigned long long both = ((unsigned long long)s0 << 32) | s1;
+  unsigned r = 0;
+  for (int i = 0; i < 4; ++i) {
+    const unsigned b = (sel >> (8 * i)) & 0xFFu;
+    assert(b < 8 || b >= 12);
+    const unsigned byte =
+        b < 8 ? (unsigned)(both >> (8 * b)) & 0xFFu : (b == 12 ? 0u : 0xFFu);
+    r |= byte << (8 * i);
+  }
+  return r;
+}
+#define __builtin_amdgcn_perm(s0, s1, sel) emu_perm(s0, s1, sel)
 // global_load_lds: lane l copies `size` bytes from its source to byte
 // off + size * l of the wave's LDS destination.  Waves see memory in program
 // order here, so waits on the copies are no-ops.

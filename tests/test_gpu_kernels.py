@@ -332,6 +332,122 @@ def _split_terms(z, form, nterms, scale=1.0):
     return ts, ldt
 
 
+def _np_term_words(re, im, f16, nterms):
+    """The term arrays the HERK reads, restated in numpy from float32 real
+    and imaginary parts (rows x columns): each term the round-to-nearest-even
+    16-bit value of the remainder the previous terms left; per 4-column
+    chunk the 16-bit real parts of the 4 columns (two a word, the lower
+    column in the low half), then their imaginary parts; columns past the
+    last one up to a multiple of 4 are zeros."""
+    def split(x):
+        out, r = [], x.astype(np.float32)
+        for _ in range(nterms):
+            if f16:
+                h = r.astype(np.float16)
+                b = h.view(np.uint16).astype(np.uint32)
+                v = h.astype(np.float32)
+            else:
+                u = r.view(np.uint32).astype(np.uint64)
+                b = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).astype(
+                    np.uint32)
+                v = (b << np.uint32(16)).view(np.float32)
+            out.append(b)
+            r = (r - v).astype(np.float32)
+        return out
+
+    m, k = re.shape
+    k4 = -(-k // 4) * 4
+    words = []
+    for tr, ti in zip(split(re), split(im)):
+        R = np.pad(tr, ((0, 0), (0, k4 - k))).reshape(m, k4 // 4, 4)
+        I = np.pad(ti, ((0, 0), (0, k4 - k))).reshape(m, k4 // 4, 4)
+        W = np.stack([R[..., 0] | R[..., 1] << 16, R[..., 2] | R[..., 3] << 16,
+                      I[..., 0] | I[..., 1] << 16, I[..., 2] | I[..., 3] << 16],
+                     -1)
+        words.append(W.reshape(m, k4))
+    return words
+
+
+@pytest.mark.parametrize("form,nterms", [(FORM_SPLIT_BF16, 2),
+                                         (FORM_SPLIT_BF16, 3),
+                                         (FORM_SPLIT_F16, 2),
+                                         (FORM_SPLIT_F16, 3)])
+@pytest.mark.parametrize("zdtype", [np.complex64, np.complex128])
+def test_split_z_terms_bit_exact(form, nterms, zdtype):
+    """mv_split_z's term words equal the numpy restatement bit for bit, on
+    values spanning 8 decades with exact zeros and both signs, 77 columns
+    (a partial last chunk) and a scale."""
+    rng = np.random.default_rng(nterms + (form == FORM_SPLIT_F16))
+    m, k = 5, 77
+    mag = np.exp(rng.uniform(-8, 0, (m, k)) * np.log(10))
+    z = (mag * np.exp(2j * np.pi * rng.uniform(size=(m, k)))).astype(zdtype)
+    z[1, 3] = 0
+    z[2, 10] = -0.75 + 0.0j
+    scale = 0.5
+    ts, ldt = _split_terms(z, form, nterms, scale)
+    re = z.real.astype(np.float32) * np.float32(scale)
+    im = z.imag.astype(np.float32) * np.float32(scale)
+    want = _np_term_words(re, im, form == FORM_SPLIT_F16, nterms)
+    k4 = want[0].shape[1]
+    for t, w in zip(ts, want):
+        got = D.to_host(t).view(np.uint32)
+        assert np.array_equal(got[:, :k4], w)
+        assert np.all(got[:, k4:] == 0)
+
+
+@pytest.mark.parametrize("polarized", [False, True])
+def test_zmat_fused_split_is_the_split_of_its_complex64(polarized):
+    """The fused kernel's split forms write exactly the terms of the
+    complex64 Z its complex64 form writes for the same inputs (the same
+    FP32 arithmetic up to the store), in the restated layout; padding
+    columns are zero terms and nothing past them is written."""
+    nant, nsrc, seg = 9, 1500, 928
+    antpos = synth.disk_array(nant, 200.0, 4)
+    eci = _sky(nsrc, 19)
+    flux = np.random.default_rng(2).uniform(0.1, 3, nsrc)
+    crd, fl, n, idx = O.select_chunk(O.rotate(_rot(), eci), np.sqrt(0.5 * flux),
+                                     0, nsrc, 900)
+    nf = 2 if polarized else 1
+    rng = np.random.default_rng(4)
+    beam = (rng.normal(size=(2, nf, nf, 900))
+            + 1j * rng.normal(size=(2, nf, nf, 900))).astype(np.complex64)
+    bufs = dict(
+        ant=_dev(antpos, np.float64),
+        crd=_dev(np.pad(crd, ((0, 0), (0, seg - 900))), np.float64),
+        flux=_dev(np.sqrt(0.5 * flux), np.float64),
+        idx=_dev(np.pad(idx, (0, seg - n)).astype(np.int32)),
+        beam=_dev(np.pad(beam, ((0, 0),) * 3 + ((0, seg - 900),))),
+        bidx=_dev((np.arange(nant) % 2).astype(np.int32)),
+        n=_dev(np.array([n], np.int32)))
+
+    def fused(form, nterms, outs):
+        zp = [D.ptr(o) for o in outs] + [None] * (3 - len(outs))
+        call("mv_zmat_fused", D.ptr(bufs["ant"]), nant, 150e6,
+             D.ptr(bufs["crd"]), seg, D.ptr(bufs["flux"]), D.ptr(bufs["idx"]),
+             D.ptr(bufs["beam"]), 0, seg, 2, D.ptr(bufs["bidx"]), nf, nf, 900,
+             D.ptr(bufs["n"]), form, nterms, 1.0, zp[0], zp[1], zp[2],
+             nf * seg, seg, D.stream_ptr())
+
+    z64 = D.zeros((nant * nf, nf * seg), np.complex64)
+    fused(FORM_C64, 1, [z64])
+    z = D.to_host(z64)
+    npad = -(-n // 32) * 32
+    for form, nterms in [(FORM_SPLIT_BF16, 2), (FORM_SPLIT_BF16, 3),
+                         (FORM_SPLIT_F16, 2)]:
+        outs = [D.zeros((nant * nf, nf * seg), np.int32) + 0x5A5A5A5A
+                for _ in range(nterms)]
+        fused(form, nterms, outs)
+        for ax in range(nf):
+            cols = slice(ax * seg, ax * seg + npad)
+            want = _np_term_words(z[:, cols].real, z[:, cols].imag,
+                                  form == FORM_SPLIT_F16, nterms)
+            for o, w in zip(outs, want):
+                got = D.to_host(o).view(np.uint32)
+                assert np.array_equal(got[:, cols], w), (form, nterms, ax)
+                assert np.all(got[:, ax * seg + npad:(ax + 1) * seg]
+                              == 0x5A5A5A5A)
+
+
 # Bounds: the split's representation error with the exact diagonal (bf16
 # two terms up to 3.4e-6 at k = 77, 3 or 4 products alike; 3 terms ~5e-9;
 # fp16 ~6e-8 on these inputs) plus FP32 accumulation within one K-slice.
