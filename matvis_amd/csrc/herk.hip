@@ -46,6 +46,18 @@ __device__ __forceinline__ void tile_coords(int t, int nb, int &bi, int &bj) {
   bj = i + t;
 }
 
+// A 64 x 64 wave sub-tile (wr, wc) of tile (bi, bj), tile side ts, that the
+// split kernels leave idle: below the diagonal of a diagonal tile, or
+// starting at or past row or column m.  Idle sub-tiles get no MFMAs and no
+// partials, and the reduce skips them: their accumulator elements lie below
+// the diagonal or past m, which the accumulator's contract leaves as scratch.
+// (Plain tiles are one 64-wide sub-tile that always starts below m.)
+__host__ __device__ inline bool subtile_idle(int bi, int bj, int wr, int wc,
+                                             int ts, int m) {
+  return (bi == bj && wr > wc) || bi * ts + wr * 64 >= m ||
+         bj * ts + wc * 64 >= m;
+}
+
 struct SliceCfg {
   int max_slices;  // partial slots per tile in the workspace
   int slots;       // workgroups resident at once (multiple of kXcd)
@@ -479,8 +491,7 @@ __device__ __forceinline__ void split_item(const SplitProblem &P, int slice,
   int bi, bj;
   tile_coords(tile, P.nb, bi, bj);
   const bool diag = bi == bj;
-  const bool active = !(diag && wr > wc) && bi * TILE + wr * 64 < P.m &&
-                      bj * TILE + wc * 64 < P.m;
+  const bool active = !subtile_idle(bi, bj, wr, wc, TILE, P.m);
   const int kt0 = (int)((int64_t)slice * ktiles / ns);
   const int kt1 = (int)((int64_t)(slice + 1) * ktiles / ns);
   // copy lanes: run g = wave * kRuns + u covers rows 16 g .. 16 g + 15;
@@ -653,6 +664,7 @@ __device__ __forceinline__ void split_item(const SplitProblem &P, int slice,
     }
   }
 
+  if (!active) return;  // an idle sub-tile has no partials
   // 32x32 accumulator layout: lane l, register r ->
   //   row 8*(r/4) + 4*(l/32) + r%4, col l%32.
   // The thread's base offset is opaque to the optimizer so that the store
@@ -829,12 +841,14 @@ template <typename TP> constexpr int kReduceEpt = 2;
 template <typename TP, bool DTILE>
 __device__ __forceinline__ void reduce_elems_t(
     const cplx<TP> *__restrict__ part, const double *__restrict__ dpart,
-    int ntiles, int bi, int bj, int ts, int tile, int64_t e, int ns,
+    int ntiles, int bi, int bj, int ts, int m, int tile, int64_t e, int ns,
     double scale, cplx<double> *__restrict__ vacc, int64_t ldv,
     int accumulate) {
   constexpr int E = kReduceEpt<TP>;
   const int64_t tsz = (int64_t)ts * ts;
   const int64_t r = e / ts, c = e % ts;
+  // the thread's E adjacent columns share a sub-tile (E divides 64)
+  if (subtile_idle(bi, bj, (int)(r >> 6), (int)(c >> 6), ts, m)) return;
   const cplx<TP> *src = part + (int64_t)tile * tsz + e;
   const int64_t step = (int64_t)ntiles * tsz;
   // diagonal tiles (DTILE) also sum the row norms of the thread's row, in
@@ -900,16 +914,17 @@ __device__ __forceinline__ void reduce_elems_t(
 template <typename TP>
 __device__ __forceinline__ void reduce_elems(
     const cplx<TP> *__restrict__ part, const double *__restrict__ dpart,
-    int ntiles, int nb, int ts, int tile, int64_t e, int ns, double scale,
-    cplx<double> *__restrict__ vacc, int64_t ldv, int accumulate) {
+    int ntiles, int nb, int ts, int m, int tile, int64_t e, int ns,
+    double scale, cplx<double> *__restrict__ vacc, int64_t ldv,
+    int accumulate) {
   int bi, bj;
   tile_coords(tile, nb, bi, bj);
   if (dpart && bi == bj)  // uniform over the block: one tile a block
-    reduce_elems_t<TP, true>(part, dpart, ntiles, bi, bj, ts, tile, e, ns,
+    reduce_elems_t<TP, true>(part, dpart, ntiles, bi, bj, ts, m, tile, e, ns,
                              scale, vacc, ldv, accumulate);
   else
-    reduce_elems_t<TP, false>(part, dpart, ntiles, bi, bj, ts, tile, e, ns,
-                              scale, vacc, ldv, accumulate);
+    reduce_elems_t<TP, false>(part, dpart, ntiles, bi, bj, ts, m, tile, e,
+                              ns, scale, vacc, ldv, accumulate);
 }
 
 // Blocks of 256 threads, kReduceEpt<TP> elements a thread, per tile.
@@ -923,7 +938,7 @@ template <typename TP> dim3 reduce_grid(int ts, int64_t ntiles) {
 template <typename TP>
 __global__ __launch_bounds__(256) void herk_reduce_kernel(
     const cplx<TP> *__restrict__ part, const double *__restrict__ dpart,
-    int ntiles, int nb, int ts, int64_t seg_len_host,
+    int ntiles, int nb, int ts, int m, int64_t seg_len_host,
     const int32_t *__restrict__ seg_len_dev, int nseg, int kc, SliceCfg cfg,
     double scale, cplx<double> *__restrict__ vacc, int64_t ldv,
     int accumulate) {
@@ -938,8 +953,8 @@ __global__ __launch_bounds__(256) void herk_reduce_kernel(
   const int64_t e =
       kReduceEpt<TP> * ((int64_t)blockIdx.x * 256 + threadIdx.x);
   if (e >= (int64_t)ts * ts) return;
-  reduce_elems(part, dpart, ntiles, nb, ts, (int)blockIdx.y, e, ns_shared,
-               scale, vacc, ldv, accumulate);
+  reduce_elems(part, dpart, ntiles, nb, ts, m, (int)blockIdx.y, e,
+               ns_shared, scale, vacc, ldv, accumulate);
 }
 
 // Grouped reduce: blockIdx.y runs over the tiles of all problems (prefix
@@ -956,7 +971,7 @@ __global__ __launch_bounds__(256) void herk_group_reduce_kernel(
   if ((int32_t)blockIdx.y >= plan.tile_off[nprob]) return;
   const int p = find_segment(plan.tile_off, nprob, (int32_t)blockIdx.y);
   const SplitProblem &P = probs[p];
-  reduce_elems((const cplx<TP> *)P.part, P.dpart, P.ntiles, P.nb, ts,
+  reduce_elems((const cplx<TP> *)P.part, P.dpart, P.ntiles, P.nb, ts, P.m,
                (int)blockIdx.y - plan.tile_off[p], e, plan.ns[p], P.scale,
                P.vacc, P.ldv, P.accumulate);
 }
@@ -1219,9 +1234,9 @@ int herk_run(int phases, int form, int nterms, int nprod, const void *z0,
     if (!(phases & kPhaseReduce)) return MV_OK;
     herk_reduce_kernel<double>
         <<<reduce_grid<double>(ts, ntiles), 256, 0, st>>>(
-        (const cplx<double> *)workspace, nullptr, p.ntiles, p.nb, ts, seg_len,
-        seg_len_dev, nseg, p.kc, p.cfg, scale, (cplx<double> *)vacc, ldv,
-        accumulate);
+        (const cplx<double> *)workspace, nullptr, p.ntiles, p.nb, ts, m,
+        seg_len, seg_len_dev, nseg, p.kc, p.cfg, scale, (cplx<double> *)vacc,
+        ldv, accumulate);
     return check_launch("mv_herk/reduce");
   }
   if (form == MV_FORM_C64) {
@@ -1234,9 +1249,9 @@ int herk_run(int phases, int form, int nterms, int nprod, const void *z0,
     if (!(phases & kPhaseReduce)) return MV_OK;
     herk_reduce_kernel<float>
         <<<reduce_grid<float>(ts, ntiles), 256, 0, st>>>(
-        (const cplx<float> *)workspace, nullptr, p.ntiles, p.nb, ts, seg_len,
-        seg_len_dev, nseg, p.kc, p.cfg, scale, (cplx<double> *)vacc, ldv,
-        accumulate);
+        (const cplx<float> *)workspace, nullptr, p.ntiles, p.nb, ts, m,
+        seg_len, seg_len_dev, nseg, p.kc, p.cfg, scale, (cplx<double> *)vacc,
+        ldv, accumulate);
     return check_launch("mv_herk/reduce");
   }
   MV_REQUIRE(z1 && (nterms < 3 || z2), "mv_herk: missing term arrays");
@@ -1289,7 +1304,7 @@ int herk_run(int phases, int form, int nterms, int nprod, const void *z0,
 #undef MV_SPLIT_LAUNCH
   if (!(phases & kPhaseReduce)) return MV_OK;
   herk_reduce_kernel<float><<<reduce_grid<float>(ts, ntiles), 256, 0, st>>>(
-      (const cplx<float> *)workspace, dpart, p.ntiles, p.nb, ts, seg_len,
+      (const cplx<float> *)workspace, dpart, p.ntiles, p.nb, ts, m, seg_len,
       seg_len_dev, nseg, p.kc, p.cfg, scale, (cplx<double> *)vacc, ldv,
       accumulate);
   return check_launch("mv_herk/reduce");
