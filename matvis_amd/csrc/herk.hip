@@ -832,34 +832,48 @@ __global__ __launch_bounds__(256) void herk_plain_grouped_kernel(
 // `dpart` (split forms) the diagonal elements of diagonal tiles take the row
 // norms summed there instead of the MFMA sums.
 //
-// A thread takes kReduceEpt adjacent elements of one row, as eight
-// independent sums over slices s = 8q + j combined in a fixed order, so
-// eight loads are in flight.  (Four elements a thread halve the grid of the
-// FP32-partial reduce to 1.3 workgroups a CU and measured slower.)
+// A wave takes 32 groups of kReduceEpt adjacent elements of one row; each
+// group's sum over slices s = 8q + j is eight independent chains j, chains
+// 0-3 in lanes 0-31 and chains 4-7 in lanes 32-63, so that each lane keeps
+// four loads in flight and a launch has twice the workgroups of one lane per
+// group.  The chains combine in a fixed order, ((c0 + c1) + (c2 + c3)) +
+// ((c4 + c5) + (c6 + c7)); lane half h then stores element h of the group.
+// (Four elements a lane halve the grid and measured slower.)
 template <typename TP> constexpr int kReduceEpt = 2;
+
+// The first element of the calling lane's group and its lane half.
+template <typename TP>
+__device__ __forceinline__ int64_t reduce_group(int &half) {
+  const int lane = (int)(threadIdx.x & 63);
+  half = lane >> 5;
+  return kReduceEpt<TP> * ((int64_t)blockIdx.x * 128 +
+                           (int64_t)(threadIdx.x >> 6) * 32 + (lane & 31));
+}
 
 template <typename TP, bool DTILE>
 __device__ __forceinline__ void reduce_elems_t(
     const cplx<TP> *__restrict__ part, const double *__restrict__ dpart,
-    int ntiles, int bi, int bj, int ts, int m, int tile, int64_t e, int ns,
-    double scale, cplx<double> *__restrict__ vacc, int64_t ldv,
+    int ntiles, int bi, int bj, int ts, int m, int tile, int64_t e, int h,
+    int ns, double scale, cplx<double> *__restrict__ vacc, int64_t ldv,
     int accumulate) {
   constexpr int E = kReduceEpt<TP>;
+  static_assert(E == 2, "lane half h stores element h of its group");
   const int64_t tsz = (int64_t)ts * ts;
   const int64_t r = e / ts, c = e % ts;
-  // the thread's E adjacent columns share a sub-tile (E divides 64)
+  // the group's E adjacent columns share a sub-tile (E divides 64); both
+  // lane halves of a group leave together
   if (subtile_idle(bi, bj, (int)(r >> 6), (int)(c >> 6), ts, m)) return;
   const cplx<TP> *src = part + (int64_t)tile * tsz + e;
   const int64_t step = (int64_t)ntiles * tsz;
-  // diagonal tiles (DTILE) also sum the row norms of the thread's row, in
-  // the same loop (every lane of the tile, so no lane runs extra rounds of
-  // dependent loads after it; a wave's lanes share two rows' addresses)
+  // diagonal tiles (DTILE) also sum the row norms of the group's row, in the
+  // same loop (every lane of the tile, so no lane runs extra rounds of
+  // dependent loads after it)
   constexpr bool dtile = DTILE;
   const double *dp = dtile ? dpart + (int64_t)tile * ts + r : nullptr;
   const int64_t dstep = (int64_t)ntiles * ts;
-  double re[E][8], im[E][8], dn[8];
+  double re[E][4], im[E][4], dn[4];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
+  for (int j = 0; j < 4; ++j) {
     dn[j] = 0.0;
 #pragma unroll
     for (int k = 0; k < E; ++k) re[k][j] = im[k][j] = 0.0;
@@ -867,12 +881,13 @@ __device__ __forceinline__ void reduce_elems_t(
   int s = 0;
   for (; s + 8 <= ns; s += 8) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const cplx<TP> *q = src + (int64_t)(s + j) * step;
+    for (int j = 0; j < 4; ++j) {
+      const int64_t sl = s + 4 * h + j;  // chain 4 h + j
+      const cplx<TP> *q = src + sl * step;
       cplx<TP> v[E];
 #pragma unroll
       for (int k = 0; k < E; ++k) v[k] = q[k];
-      if constexpr (dtile) dn[j] += dp[(int64_t)(s + j) * dstep];
+      if constexpr (dtile) dn[j] += dp[sl * dstep];
 #pragma unroll
       for (int k = 0; k < E; ++k) {
         re[k][j] += (double)v[k].re;
@@ -880,57 +895,71 @@ __device__ __forceinline__ void reduce_elems_t(
       }
     }
   }
-  for (; s < ns; ++s) {
-    const cplx<TP> *q = src + (int64_t)s * step;
+  if (h == 0) {
+    // the last ns % 8 slices join chain 0
+    for (; s < ns; ++s) {
+      const cplx<TP> *q = src + (int64_t)s * step;
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-      re[k][0] += (double)q[k].re;
-      im[k][0] += (double)q[k].im;
+      for (int k = 0; k < E; ++k) {
+        re[k][0] += (double)q[k].re;
+        im[k][0] += (double)q[k].im;
+      }
+      if constexpr (dtile) dn[0] += dp[(int64_t)s * dstep];
     }
-    if constexpr (dtile) dn[0] += dp[(int64_t)s * dstep];
   }
-  cplx<double> *dst = vacc + ((int64_t)bi * ts + r) * ldv + (int64_t)bj * ts + c;
+  // this half's four chains, then (chains 0-3) + (chains 4-7) in both
+  // halves (the same sum: an IEEE add commutes)
+  auto both = [&](const double (&x)[4]) {
+    const double a = (x[0] + x[1]) + (x[2] + x[3]);
+    const double b = __shfl_xor(a, 32, 64);
+    return h == 0 ? a + b : b + a;
+  };
+  double sr[E], si[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) {
-    double sr = ((re[k][0] + re[k][1]) + (re[k][2] + re[k][3])) +
-                ((re[k][4] + re[k][5]) + (re[k][6] + re[k][7]));
-    double si = ((im[k][0] + im[k][1]) + (im[k][2] + im[k][3])) +
-                ((im[k][4] + im[k][5]) + (im[k][6] + im[k][7]));
-    if (dtile && r == c + k) {
+    sr[k] = both(re[k]);
+    si[k] = both(im[k]);
+  }
+  double vr = h == 0 ? sr[0] : sr[1];
+  double vi = h == 0 ? si[0] : si[1];
+  if constexpr (dtile) {
+    const double d = both(dn);
+    if (r == c + h) {
       // the diagonal: the FP64 sum of |terms|^2 the compute kernel kept
-      sr = ((dn[0] + dn[1]) + (dn[2] + dn[3])) +
-           ((dn[4] + dn[5]) + (dn[6] + dn[7]));
-      si = 0.0;
+      vr = d;
+      vi = 0.0;
     }
-    if (accumulate) {
-      dst[k].re += sr * scale;
-      dst[k].im += si * scale;
-    } else {
-      dst[k] = cplx<double>{sr * scale, si * scale};
-    }
+  }
+  cplx<double> *dst =
+      vacc + ((int64_t)bi * ts + r) * ldv + (int64_t)bj * ts + c + h;
+  if (accumulate) {
+    dst->re += vr * scale;
+    dst->im += vi * scale;
+  } else {
+    *dst = cplx<double>{vr * scale, vi * scale};
   }
 }
 
 template <typename TP>
 __device__ __forceinline__ void reduce_elems(
     const cplx<TP> *__restrict__ part, const double *__restrict__ dpart,
-    int ntiles, int nb, int ts, int m, int tile, int64_t e, int ns,
+    int ntiles, int nb, int ts, int m, int tile, int64_t e, int h, int ns,
     double scale, cplx<double> *__restrict__ vacc, int64_t ldv,
     int accumulate) {
   int bi, bj;
   tile_coords(tile, nb, bi, bj);
   if (dpart && bi == bj)  // uniform over the block: one tile a block
-    reduce_elems_t<TP, true>(part, dpart, ntiles, bi, bj, ts, m, tile, e, ns,
-                             scale, vacc, ldv, accumulate);
+    reduce_elems_t<TP, true>(part, dpart, ntiles, bi, bj, ts, m, tile, e, h,
+                             ns, scale, vacc, ldv, accumulate);
   else
-    reduce_elems_t<TP, false>(part, dpart, ntiles, bi, bj, ts, m, tile, e,
+    reduce_elems_t<TP, false>(part, dpart, ntiles, bi, bj, ts, m, tile, e, h,
                               ns, scale, vacc, ldv, accumulate);
 }
 
-// Blocks of 256 threads, kReduceEpt<TP> elements a thread, per tile.
+// Blocks of 256 threads (128 element groups), per tile.
 template <typename TP> dim3 reduce_grid(int ts, int64_t ntiles) {
   return dim3((unsigned)cdiv((int64_t)ts * ts,
-                             (int64_t)256 * kReduceEpt<TP>),
+                             (int64_t)128 * kReduceEpt<TP>),
               (unsigned)ntiles);
 }
 
@@ -950,21 +979,21 @@ __global__ __launch_bounds__(256) void herk_reduce_kernel(
     if (threadIdx.x == 0) ns_shared = ns;
   }
   __syncthreads();
-  const int64_t e =
-      kReduceEpt<TP> * ((int64_t)blockIdx.x * 256 + threadIdx.x);
+  int h;
+  const int64_t e = reduce_group<TP>(h);
   if (e >= (int64_t)ts * ts) return;
-  reduce_elems(part, dpart, ntiles, nb, ts, m, (int)blockIdx.y, e,
+  reduce_elems(part, dpart, ntiles, nb, ts, m, (int)blockIdx.y, e, h,
                ns_shared, scale, vacc, ldv, accumulate);
 }
 
 // Grouped reduce: blockIdx.y runs over the tiles of all problems (prefix
-// offsets from the plan); elements per thread as above.
+// offsets from the plan); lanes and element groups as above.
 template <typename TP>
 __global__ __launch_bounds__(256) void herk_group_reduce_kernel(
     const SplitProblem *__restrict__ probs, int nprob, int ts,
     GroupPlan plan) {
-  const int64_t e =
-      kReduceEpt<TP> * ((int64_t)blockIdx.x * 256 + threadIdx.x);
+  int h;
+  const int64_t e = reduce_group<TP>(h);
   if (e >= (int64_t)ts * ts) return;
   // the grid covers the tiles of the problems' largest shapes (a graph
   // replayed with smaller ones keeps it); tiles past this plan's total idle
@@ -972,7 +1001,7 @@ __global__ __launch_bounds__(256) void herk_group_reduce_kernel(
   const int p = find_segment(plan.tile_off, nprob, (int32_t)blockIdx.y);
   const SplitProblem &P = probs[p];
   reduce_elems((const cplx<TP> *)P.part, P.dpart, P.ntiles, P.nb, ts, P.m,
-               (int)blockIdx.y - plan.tile_off[p], e, plan.ns[p], P.scale,
+               (int)blockIdx.y - plan.tile_off[p], e, h, plan.ns[p], P.scale,
                P.vacc, P.ldv, P.accumulate);
 }
 

@@ -112,6 +112,33 @@ def test_multichunk_source_buffer_and_antpairs():
     np.testing.assert_allclose(v1, v, rtol=0, atol=1e-12 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("method,terms,bound", [
+    ("GPUMatMulSplitBF16", 2, 2e-6), ("GPUMatMulSplitBF16", 3, 1e-6),
+    ("GPUMatMulSplitF16", 2, 1e-6)])
+def test_split_forms_multichunk_beams_and_pairs(method, terms, bound):
+    """The split forms through the drop-in path with 3 source chunks (each
+    chunk's HERK adds into the FP64 accumulator; split-FP16 scales each
+    chunk by its own power of two), a source buffer and two grid beams
+    assigned per antenna, against the oracle; a pair subset equals the same
+    pairs of the full output bit for bit."""
+    nant = 12
+    p = small_problem(nant=nant, nsrc=3001, ntimes=2, polarized=True,
+                      beam="grid", order=3, nbeam=2)
+    bidx = np.arange(nant) % 2
+    kw = dict(matprod_method=method, split_terms=terms, min_chunks=3,
+              source_buffer=0.8, beam_idx=bidx)
+    v = run_gpu(p, **kw)
+    ref = run_oracle(p, nchunks=3, source_buffer=0.8, beam_idx=bidx)
+    e = nerr(v, ref, nant, True)
+    print(f"{method} ({terms} terms), 3 chunks: normalised max {e[0]:.2e}",
+          flush=True)
+    assert e[0] < bound
+    pairs = np.array([(0, 0), (3, 7), (7, 3), (11, 1), (2, 2)])
+    vp = run_gpu(p, antpairs=pairs, **kw)
+    full = v.reshape(v.shape[0], nant, nant, 2, 2)
+    assert np.array_equal(vp, full[:, pairs[:, 0], pairs[:, 1]])
+
+
 def test_source_buffer_overflow_raises():
     p = small_problem(nant=4, nsrc=3000, ntimes=1)
     with pytest.raises(ValueError, match="is too small for the number of"):
