@@ -154,6 +154,29 @@ int mv_beam_interp(const void *planes, int is_f64, int nbeam, int nfeed,
                    int64_t ld_crd, int64_t count, const int32_t *n_dev,
                    void *out, int64_t ld_out, mv_stream_t stream);
 
+/* mv_beam_interp with the grid staged in LDS by za band instead of tap
+ * gathers.  On MI355X it measured slower than mv_beam_interp at every grid
+ * whose rows fit (the planes of a 0.25 deg beam sit in the 256 MB Infinity
+ * Cache), so the library uses it only when asked.
+ * The positions are binned by za band (B grid rows; B + 3 rows of one plane,
+ * 128 KB at most, staged per workgroup), and each result is written to its
+ * own position's slot, so the output equals mv_beam_interp's when every
+ * beam shares beam 0's za spacing grid[1] (the bands use it).  Replaces the
+ * same reference call as mv_beam_interp.  workspace: device memory of the
+ * size the next function returns for (is_f64, nza, naz, count); that is 0
+ * when a band's rows would not fit in LDS, and then the gather kernel runs
+ * and no workspace is needed.  count <= INT32_MAX.  Other arguments as
+ * mv_beam_interp. */
+size_t mv_beam_interp_banded_workspace_bytes(int is_f64, int nza, int naz,
+                                             int64_t count);
+int mv_beam_interp_banded(const void *planes, int is_f64, int nbeam,
+                          int nfeed, int nax, int nza, int naz,
+                          const double *grid, int order, int is_power,
+                          const void *crd, int crd_is_f64, int64_t ld_crd,
+                          int64_t count, const int32_t *n_dev, void *out,
+                          int64_t ld_out, void *workspace, size_t ws_bytes,
+                          mv_stream_t stream);
+
 /* As mv_beam_interp, but at given (az, za) arrays (f64, count) instead of
  * topocentric vectors: the reference's gpu_beam_interpolation(beam, daz, dza,
  * azmin, az, za) utility (src/matvis/gpu/beams.py:127-200). */

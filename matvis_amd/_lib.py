@@ -48,6 +48,9 @@ SIGNATURES: dict[str, list] = {
                        _L, _P, _P, _L, _P],
     "mv_beam_interp_azza": [_P, _I, _I, _I, _I, _I, _I, _P, _I, _I, _P, _P,
                             _L, _P, _L, _P],
+    "mv_beam_interp_banded_workspace_bytes": [_I, _I, _I, _L],
+    "mv_beam_interp_banded": [_P, _I, _I, _I, _I, _I, _I, _P, _I, _I, _P, _I,
+                              _L, _L, _P, _P, _L, _P, _S, _P],
     "mv_beam_gaussian": [_P, _I, _I, _P, _I, _L, _L, _P, _P, _I, _L, _P],
     "mv_exptau": [_P, _I, _D, _P, _I, _L, _L, _P, _I, _L, _P],
     "mv_zmat_parts": [_P, _P, _P, _I, _P, _I, _I, _I, _L, _P, _I, _P],
@@ -76,6 +79,7 @@ SIGNATURES: dict[str, list] = {
 
 _RESTYPES = {
     "mv_rotate_cull_workspace_bytes": _S,
+    "mv_beam_interp_banded_workspace_bytes": _S,
     "mv_herk_workspace_bytes": _S,
     "mv_herk_problem_bytes": _S,
     "mv_herk_grouped_workspace_bytes": _S,

@@ -1,9 +1,12 @@
 """GPU beam evaluation (matvis gpu/beams.py:15-200) on HIP kernels.
 
 Grid beams are interpolated by ``mv_beam_interp`` (bilinear, or bicubic
-B-spline on planes prefiltered once by ``mv_beam_prefilter``); Gaussian
-analytic beams are evaluated on the device by ``mv_beam_gaussian`` instead of
-the reference's host round trip (gpu/beams.py:95-101).
+B-spline on planes prefiltered once by ``mv_beam_prefilter``), whose taps
+gather from L2 and the Infinity Cache; ``mv_beam_interp_banded`` stages the
+grid in LDS by za band instead (the same values, opt-in: it measured slower
+on every grid it fits).  Gaussian analytic beams are evaluated on the device
+by ``mv_beam_gaussian`` instead of the reference's host round trip
+(gpu/beams.py:95-101).
 """
 
 from __future__ import annotations
@@ -12,7 +15,7 @@ import numpy as np
 import torch
 
 from .. import _device as D
-from .._lib import call
+from .._lib import call, lib
 from ..beams import grid_planes, is_analytic_beam, is_grid_beam, unwrap
 from ..core.beams import BeamInterpolator
 
@@ -34,7 +37,11 @@ class DeviceBeamSet:
     """Device-resident tables of a beam list at one frequency."""
 
     def __init__(self, beam_list, freq: float, polarized: bool, complex_dtype,
-                 order: int = 1):
+                 order: int = 1, banded: bool = False):
+        """``banded``: interpolate with the grid staged in LDS by za band
+        (the same values as the default tap gathers, which measured faster
+        on MI355X for every grid whose rows fit in LDS); the bands need
+        every beam to share beam 0's za spacing."""
         self.nbeam = len(beam_list)
         self.nfeed = self.nax = 2 if polarized else 1
         self.polarized = polarized
@@ -68,17 +75,34 @@ class DeviceBeamSet:
                 call("mv_beam_prefilter", D.ptr(self.planes), int(self.is_f64),
                      self.nbeam * self.nax * self.nfeed, self.nza, self.naz,
                      D.stream_ptr())
+            if banded and not np.all(grid[:, 1] == grid[0, 1]):
+                raise ValueError("the za-band path needs every beam to share "
+                                 "one za spacing")
+            self.banded = bool(banded)
+            self._band_ws = None
         else:
             sig = [unwrap(b)[0].get_sigma(freq) for b in beam_list]
             self.sigma = D.to_dev(np.asarray(sig, np.float64))
             self.is_power = False
             self.max_abs = 1.0
+            self.banded = False
 
     def eval(self, crd, ld_crd: int, count: int, out, ld_out: int,
              n_dev=None, stream=None):
         """A (nbeam, nfeed, nax, ld_out) at topocentric vectors crd (3, ld)."""
         crd_is_f64 = int(crd.dtype == torch.float64)
-        if self.use_interp:
+        if self.use_interp and self.banded:
+            need = lib().mv_beam_interp_banded_workspace_bytes(
+                int(self.is_f64), self.nza, self.naz, count)
+            if self._band_ws is None or self._band_ws.numel() < need:
+                self._band_ws = D.empty((max(need, 1),), np.uint8)
+            call("mv_beam_interp_banded", D.ptr(self.planes),
+                 int(self.is_f64), self.nbeam, self.nfeed, self.nax, self.nza,
+                 self.naz, D.ptr(self.grid), self.order, int(self.is_power),
+                 D.ptr(crd), crd_is_f64, ld_crd, count, D.ptr(n_dev),
+                 D.ptr(out), ld_out, D.ptr(self._band_ws),
+                 self._band_ws.numel(), D.stream_ptr(stream))
+        elif self.use_interp:
             call("mv_beam_interp", D.ptr(self.planes), int(self.is_f64),
                  self.nbeam, self.nfeed, self.nax, self.nza, self.naz,
                  D.ptr(self.grid), self.order, int(self.is_power), D.ptr(crd),

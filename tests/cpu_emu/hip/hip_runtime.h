@@ -175,6 +175,10 @@ inline void emu_global_load_lds(const void *src, void *dst, unsigned size) {
 #define __builtin_amdgcn_s_waitcnt(imm) ((void)0)
 #define __builtin_amdgcn_s_setprio(p) ((void)0)
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+// blocks run one at a time here; the atomic keeps the same contract anyway
+inline int atomicAdd(int *p, int v) {
+  return __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
+}
 inline unsigned __float_as_uint(float f) {
   unsigned u;
   memcpy(&u, &f, 4);

@@ -53,6 +53,15 @@ def test_tiles_and_workspace():
         2 * 21 * 128 * 128 * 8 + 2 * 21 * 128 * 8)
     assert lib.mv_rotate_cull_workspace_bytes(0) == 256
     assert lib.mv_rotate_cull_workspace_bytes(2048 * 70) == 512
+    # za bands: complex64 rows of 363 -> 128 KB holds 45 rows, bands of
+    # 42; 91 rows -> 3 bands: counts, band starts, workgroup starts (256 B
+    # each, aligned), then (az, za) and band, rank, order per position
+    n = 60000
+    per = lambda b: -(-b // 256) * 256  # noqa: E731
+    assert lib.mv_beam_interp_banded_workspace_bytes(0, 91, 363, n) == (
+        3 * 256 + 2 * per(8 * n) + 3 * per(4 * n))
+    # complex128 rows of 4100 do not fit 4 in 128 KB: the gather path
+    assert lib.mv_beam_interp_banded_workspace_bytes(1, 91, 4100, n) == 0
 
 
 def _err():
@@ -74,6 +83,10 @@ def test_argument_errors_do_not_launch():
     rc = lib.mv_beam_interp(dummy, 1, 1, 1, 1, 4, 4, dummy, 2, 0, dummy, 1, 4,
                             4, None, dummy, 4, None)
     assert rc == _lib.MV_ERR_ARG and "order must be 1 or 3" in _err()
+    rc = lib.mv_beam_interp_banded(dummy, 0, 1, 2, 2, 91, 363, dummy, 3, 0,
+                                   dummy, 1, 100, 100, None, dummy, 100,
+                                   dummy, 0, None)
+    assert rc == _lib.MV_ERR_ARG and "workspace too small" in _err()
     rc = lib.mv_zmat_fused(dummy, 4, 1e8, dummy, 8, dummy, None, dummy, 1, 8,
                            1, None, 2, 2, 8, None, 0, 1, 1.0, dummy, None,
                            None, 16, 8, None)
