@@ -624,6 +624,63 @@ def host_output_rate(eng, ntimes: int, vis_per_time: int) -> dict:
                 output_bytes=int(out.nbytes))
 
 
+def beam_interp_bench(count: int = 59700, reps: int = 50) -> list:
+    """Bicubic interpolation of a polarized beam (4 planes) at `count`
+    random directions over the upper hemisphere: the tap-gather kernel
+    (mv_beam_interp) against the za bands staged in LDS
+    (mv_beam_interp_banded), at 1, 0.5 and 0.25 deg grids in complex64 and
+    complex128; HIP-event means over `reps` launches after 3 warm ones."""
+    import torch
+
+    import matvis_amd._device as D
+    import matvis_amd._lib as L
+
+    rng = np.random.default_rng(3)
+    za = np.arccos(rng.uniform(0.0, 1.0, count))
+    az = rng.uniform(0, 2 * np.pi, count)
+    crd = D.to_dev(np.stack([np.sin(za) * np.sin(az),
+                             np.sin(za) * np.cos(az), np.cos(za)]), np.float64)
+    rows = []
+    for naz, nza in ((363, 91), (723, 181), (1443, 361)):
+        for is_f64 in (0, 1):
+            dt = np.complex128 if is_f64 else np.complex64
+            planes = D.to_dev((rng.normal(size=(1, 2, 2, nza, naz))
+                               + 1j * rng.normal(size=(1, 2, 2, nza, naz)))
+                              .astype(dt))
+            grid = D.to_dev(np.array([[2 * np.pi / (naz - 3),
+                                       (np.pi / 2) / (nza - 1), 0.0]]),
+                            np.float64)
+            out = D.zeros((4, count), dt)
+            need = L.lib().mv_beam_interp_banded_workspace_bytes(
+                is_f64, nza, naz, count)
+            ws = D.empty((max(need, 1),), np.uint8)
+            args = (D.ptr(planes), is_f64, 1, 2, 2, nza, naz, D.ptr(grid), 3,
+                    0, D.ptr(crd), 1, count, count, None, D.ptr(out), count)
+            row = {"grid": f"{naz}x{nza}", "dtype": "c128" if is_f64 else "c64",
+                   "planes_mb": planes.numel() * planes.element_size() / 1e6}
+            for name in ("gather", "banded"):
+                def run():
+                    if name == "gather":
+                        L.call("mv_beam_interp", *args, D.stream_ptr())
+                    else:
+                        L.call("mv_beam_interp_banded", *args, D.ptr(ws),
+                               ws.numel(), D.stream_ptr())
+                for _ in range(3):
+                    run()
+                torch.cuda.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    run()
+                e1.record()
+                torch.cuda.synchronize()
+                row[f"{name}_us"] = 1e3 * e0.elapsed_time(e1) / reps
+            log(f"beam interpolation {row}")
+            rows.append(row)
+    return rows
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -835,6 +892,9 @@ def main():
                     help="print per-kernel HBM traffic (JSON) from two "
                     "rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of a "
                     "command that included --copy-probe, and exit")
+    ap.add_argument("--beam-interp", action="store_true",
+                    help="time beam interpolation, tap gathers against za "
+                         "bands staged in LDS, print JSON and exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy first (the "
                          "known-size kernel that calibrates HBM byte counters)")
@@ -862,6 +922,9 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.beam_interp:
+        print(json.dumps({"beam_interp": beam_interp_bench()}))
+        return
     if args.serve_sweep:
         rates = [float(x) for x in args.serve_sweep.split(",")]
         print(json.dumps({"serving_sweep": serving_sweep(
