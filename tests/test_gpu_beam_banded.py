@@ -30,7 +30,7 @@ def _positions(count, seed):
     return np.stack([e, n, np.cos(za)])
 
 
-@pytest.mark.parametrize("naz,nza", [(363, 91), (1443, 181)])
+@pytest.mark.parametrize("naz,nza", [(363, 91), (1443, 181), (4, 2)])
 @pytest.mark.parametrize("is_f64", [0, 1])
 @pytest.mark.parametrize("order", [1, 3])
 @pytest.mark.parametrize("polarized,is_power", [(True, 0), (False, 0),
@@ -46,7 +46,7 @@ def test_banded_equals_gather(naz, nza, is_f64, order, polarized, is_power):
     dza = np.radians(80.0) / (nza - 1)
     daz = 2 * np.pi / (naz - 3)
     grid = np.array([[daz, dza, 0.0], [daz, dza, -2e-3]])
-    count, n = 5000, 4711
+    count, n = (5000, 4711) if nza > 2 else (7, 5)
     crd = _positions(count, naz)
     bufs = dict(planes=D.to_dev(planes), grid=D.to_dev(grid, np.float64),
                 crd=D.to_dev(crd, np.float64),
@@ -67,7 +67,8 @@ def test_banded_equals_gather(naz, nza, is_f64, order, polarized, is_power):
         a, b = D.to_host(outs[0]), D.to_host(outs[1])
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), crd_key
         assert np.all(a[:, n:] == 0)
-        assert np.count_nonzero(a[:, :n]) > 0.7 * nplanes * n
+        if nza > 2:
+            assert np.count_nonzero(a[:, :n]) > 0.7 * nplanes * n
 
 
 def test_large_beam_matches_oracle_and_bands_match_gathers():
