@@ -241,10 +241,13 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(
 
 // ---------------------------------------------------- za bands staged in LDS
 //
-// For grids larger than an XCD's L2, the sources are binned by za band and
-// each workgroup stages its band's rows of one plane in LDS before
-// interpolating: the grid is read about once per band instead of by every
-// source's scattered taps.  Band b holds the sources whose first za tap row
+// The alternative to interp_kernel's tap gathers: the sources are binned
+// by za band and each workgroup stages its band's rows of one plane in LDS,
+// then interpolates up to kBandChunk of the band's sources from there (the
+// rows are read once a workgroup instead of by every source's scattered
+// taps; on MI355X the gathers, served by L2 and the Infinity Cache, measured
+// faster at every grid size whose rows fit, so callers ask for this path
+// explicitly).  Band b holds the sources whose first za tap row
 // floor(cx) lies in [b B, (b + 1) B), B = band_rows(naz): their bicubic taps
 // reach rows b B - 1 .. b B + B + 1 (bilinear b B .. b B + B), the B + 3
 // rows staged (folded at the grid's ends as the gather kernel folds them).
