@@ -16,14 +16,19 @@ depends only on time, so it is computed once per time instead of once per
 
 from __future__ import annotations
 
+import logging
 import os
+import time
 from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
+import psutil
 import torch
 
 from .. import _device as D
 from .._lib import FORM_C128, FORM_SPLIT_F16, call, lib
+from .._utils import log_progress
+from .._utils import logger as _progress_log
 from .beams import DeviceBeamSet
 from .herk import Herk, cdiv, expand, is_split
 from .matprod import f16_scale
@@ -341,6 +346,13 @@ class VisibilityEngine:
             else:
                 list(pool.map(lambda b: copy_block(dst, src, b), blocks))
 
+        # progress at INFO about every 1% of the times, as the reference's
+        # time loop reports (gpu/gpu.py)
+        report = ntimes // 100 + 1
+        proc = (psutil.Process() if _progress_log.isEnabledFor(logging.INFO)
+                else None)
+        t_start = t_last = time.time()
+        mem_last = proc.memory_info().rss if proc is not None else 0
         try:
             for t in range(ntimes):
                 k = t % 2
@@ -356,6 +368,9 @@ class VisibilityEngine:
                 if pending is not None:
                     finish(pending)
                 pending = (t, k)
+                if proc is not None and (t % report == 0 or t == ntimes - 1):
+                    t_last, mem_last = log_progress(t_start, t_last, t + 1,
+                                                    ntimes, proc, mem_last)
             if pending is not None:
                 finish(pending)
         finally:

@@ -119,3 +119,17 @@ def test_one_source_matches_oracle(form):
     e = _nerr(v, ref, 9, True)
     print(f"one source {form}: normalised max {e[0]:.2e}", flush=True)
     assert e[0] < bound
+
+
+def test_progress_logged_at_info(caplog):
+    """The time loop reports progress at INFO as the reference's does
+    (about every 1% of the times and at the last)."""
+    import logging
+
+    p = small_problem(nant=3, nsrc=40, ntimes=3, polarized=False)
+    with caplog.at_level(logging.INFO, logger="matvis_amd._utils"):
+        _run(p, **FORMS["fp64"][0])
+    msgs = [r.getMessage() for r in caplog.records
+            if r.name == "matvis_amd._utils"]
+    assert any("3/3 times (100.0%)" in m for m in msgs), msgs
+    assert len([m for m in msgs if "times (" in m]) == 3

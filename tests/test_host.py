@@ -19,8 +19,16 @@ def test_required_chunks_monotone_and_validated():
     assert big == 1
     counts = [get_required_chunks(mem, *args) for mem in (2**34, 2**31, 2**29)]
     assert counts == sorted(counts)
-    with pytest.raises(ValueError, match=r"memory_buffer must be in the range"):
-        get_required_chunks(1e9, *args, memory_buffer=1.5)
+    for bad in (1.5, 1.1, 0.0, -0.5):
+        with pytest.raises(ValueError, match=r"memory_buffer must be in the range"):
+            get_required_chunks(1e9, *args, memory_buffer=bad)
+    # memory_buffer = 1 is valid; a smaller buffer never needs fewer chunks;
+    # the count is clamped to 1 when everything fits
+    small = (2, 2, 4, 100, 2, 0, 1)
+    assert get_required_chunks(10 * 1024**3, *small, memory_buffer=1.0) >= 1
+    assert (get_required_chunks(2**20, *args, memory_buffer=0.1)
+            >= get_required_chunks(2**20, *args, memory_buffer=0.9))
+    assert get_required_chunks(1e18, *small, memory_buffer=1.0) == 1
     n, per = get_desired_chunks(np.inf, 3, [], 1, 1, 8, 10, 2)
     assert (n, per) == (3, 4)
     n, per = get_desired_chunks(np.inf, 50, [], 1, 1, 8, 10, 2)
@@ -348,3 +356,38 @@ def test_chunks_fit_this_builds_footprint():
     # a budget nothing fits: the cap, not an endless loop
     assert fit_chunks(1e6, 1, form=FORM_C128, nterms=1, nprod=1,
                       **kw)[0] == 1000
+
+
+def test_human_readable_size_and_ceildiv():
+    """The reference's expected strings (tests/test_utils.py:20-38)."""
+    from matvis_amd._utils import ceildiv, human_readable_size as h
+
+    assert h(0) == "0.00 B" and h(1) == "1.00 B" and h(1023) == "1023.00 B"
+    assert h(1024) == "1.00 KiB" and h(1024**2) == "1.00 MiB"
+    assert h(1024**3) == "1.00 GiB" and h(1024**4) == "1.00 TiB"
+    assert h(1024**5) == "1.00 PiB" and h(1024**6) == "1024.00 PiB"
+    assert h(1024**6, decimal_places=3) == "1024.000 PiB"
+    assert h(1024**6, decimal_places=3, indicate_sign=True) == "+1024.000 PiB"
+    assert h(-2048, indicate_sign=True) == "-2.00 KiB"
+    assert [ceildiv(a, 4) for a in (0, 1, 4, 5, 8)] == [0, 1, 1, 2, 2]
+
+
+def test_log_progress_reports_at_info(caplog):
+    import logging
+
+    from matvis_amd._utils import log_progress
+
+    class Proc:
+        def memory_info(self):
+            class M:
+                rss = 3 * 1024**3
+            return M()
+
+    with caplog.at_level(logging.WARNING, logger="matvis_amd._utils"):
+        assert log_progress(0.0, 1.0, 2, 10, Proc(), 5) == (1.0, 5)
+    assert not caplog.records
+    with caplog.at_level(logging.INFO, logger="matvis_amd._utils"):
+        t, rss = log_progress(0.0, 0.0, 2, 10, Proc(), 2 * 1024**3)
+    assert rss == 3 * 1024**3 and t > 0
+    msg = caplog.records[-1].getMessage()
+    assert "2/10 times (20.0%)" in msg and "3.00 GiB (+1.00 GiB)" in msg
