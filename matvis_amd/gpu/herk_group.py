@@ -116,14 +116,18 @@ class HerkGroup:
     def update(self, problems):
         """New contents for the same problem shapes (addresses, scales,
         accumulate flags); the device copy is rewritten in place, so a
-        captured graph replays with it."""
+        captured graph replays with it.  The copy is asynchronous on the
+        current stream, from pinned memory that torch's host allocator keeps
+        until the copy has run: the host does not wait for the work queued
+        before it."""
         host = self._pack(problems)
         for f in ("m", "nseg", "ldz", "seg_stride", "seg_len", "ldv",
                   "nslices", "kchain"):
             if not np.array_equal(host[f], self.host[f]):
                 raise ValueError(f"grouped problems changed shape ({f})")
         self.problems, self.host = problems, host
-        self.dev.copy_(torch.from_numpy(host.view(np.uint8).copy()))
+        staging = torch.from_numpy(host.view(np.uint8).copy()).pin_memory()
+        self.dev.copy_(staging, non_blocking=True)
 
     def run(self, stream=None):
         call("mv_herk_grouped", self.form, self.nterms, self.nprod, self.tile,
