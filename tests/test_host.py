@@ -391,3 +391,31 @@ def test_log_progress_reports_at_info(caplog):
     assert rss == 3 * 1024**3 and t > 0
     msg = caplog.records[-1].getMessage()
     assert "2/10 times (20.0%)" in msg and "3.00 GiB (+1.00 GiB)" in msg
+
+
+def test_autotune_classes_signature_and_cache(tmp_path, monkeypatch):
+    """Problem classes (m to a power of two, K to a power of four), batch
+    signatures independent of order, default configurations, the
+    per-problem assignment and the file cache."""
+    from matvis_amd._lib import FORM_C128, FORM_SPLIT_BF16
+    from matvis_amd.gpu import autotune as A
+
+    pc = A.problem_class
+    assert pc({"m": 700, "nseg": 2, "seg_len": 59700}) == (1024, 4 ** 9)
+    assert pc({"m": 64, "nseg": 1, "seg_len": 4}) == (64, 4)
+    assert pc({"m": 65, "nseg": 1, "seg_len": 5}) == (128, 16)
+    assert pc({"m": 1, "nseg": 1, "seg_len": 0}) == (1, 1)
+    ps = [{"m": 700, "nseg": 2, "seg_len": 59700},
+          {"m": 64, "nseg": 1, "seg_len": 4000}]
+    sig = A.signature(FORM_SPLIT_BF16, 2, 3, ps)
+    assert sig == A.signature(FORM_SPLIT_BF16, 2, 3, ps[::-1])
+    assert sig != A.signature(FORM_SPLIT_BF16, 2, 4, ps)
+    assert A.default_config(FORM_SPLIT_BF16, (64, 4096)) == (64, 192)
+    assert A.default_config(FORM_SPLIT_BF16, (1024, 4 ** 9)) == (128, 192)
+    assert A.default_config(FORM_C128, (1024, 16)) == (64, 64)
+    cfg = {pc(ps[0]): (128, 96), pc(ps[1]): (64, 48)}
+    assert A.assignment(cfg, ps) == ([128, 64], [96, 48])
+    monkeypatch.setenv("MATVIS_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    key = A.signature(FORM_SPLIT_BF16, 2, 4, ps)
+    A._store_file(key, cfg)
+    assert A._cached(key) == cfg
