@@ -178,7 +178,7 @@ def batched_bench(ninst: int = 256, seed: int = 7, reps: int = 5):
     import matvis_amd._device as D
     import matvis_amd._lib as L
     from matvis_amd.gpu import autotune
-    from matvis_amd.gpu.herk import Herk
+    from matvis_amd.gpu.herk import Herk, split_terms_alloc
     from matvis_amd.gpu.herk_group import GroupedBatch
 
     rng = np.random.default_rng(seed)
@@ -194,12 +194,12 @@ def batched_bench(ninst: int = 256, seed: int = 7, reps: int = 5):
         z = torch.randn((m, nf * seg), dtype=torch.complex64, device="cuda",
                         generator=gen)
         z.view(m, nf, seg)[:, :, nup:] = 0
-        terms = [D.zeros((m, nf * seg), np.int32) for _ in range(nt)]
+        terms = split_terms_alloc(nt, m, nf * seg)  # m rows a K tile
         L.call("mv_split_z", D.ptr(z), 0, m, nf, nup, nf * seg, seg, form, nt,
-               1.0, D.ptr(terms[0]), D.ptr(terms[1]), None, nf * seg, seg,
+               1.0, D.ptr(terms[0]), D.ptr(terms[1]), None, m, seg,
                D.stream_ptr())
         ldv = -(-m // 128) * 128
-        probs.append(dict(terms=terms, m=m, ldz=nf * seg, nseg=nf,
+        probs.append(dict(terms=terms, m=m, ldz=m, nseg=nf,
                           seg_stride=seg, seg_len=nup,
                           vacc=D.zeros((ldv, ldv), np.complex128), ldv=ldv,
                           scale=1.0, accumulate=False))
@@ -247,10 +247,13 @@ def batched_bench(ninst: int = 256, seed: int = 7, reps: int = 5):
     def herk_for(f, m, n1, n2):
         return herks.setdefault((f, m), Herk(f, m, n1, n2))
 
+    def zld(p):  # row stride of the complex Z (the terms' ldz is m)
+        return p["nseg"] * p["seg_stride"]
+
     def plain_all():
         for p, z in zip(probs, zs):
             h = herk_for(L.FORM_C64, p["m"], 1, 1)
-            h.run([z], ldz=p["ldz"], nseg=p["nseg"], seg_stride=p["seg_stride"],
+            h.run([z], ldz=zld(p), nseg=p["nseg"], seg_stride=p["seg_stride"],
                   seg_len=p["seg_len"], vacc=p["vacc"], accumulate=False,
                   nslices=h.nslices(p["nseg"] * p["seg_len"]))
 
@@ -266,7 +269,7 @@ def batched_bench(ninst: int = 256, seed: int = 7, reps: int = 5):
     t_split = sequential(split_all)
     # the plain complex64 HERKs of the same instances as one grouped launch
     from matvis_amd.gpu.herk_group import plain_tile
-    probs64 = [dict(p, terms=[z]) for p, z in zip(probs, zs)]
+    probs64 = [dict(p, terms=[z], ldz=zld(p)) for p, z in zip(probs, zs)]
     batch64 = GroupedBatch(L.FORM_C64, 1, 1, probs64,
                            [plain_tile()] * ninst).capture()
     t_group64 = sequential(batch64.replay)
@@ -279,7 +282,7 @@ def batched_bench(ninst: int = 256, seed: int = 7, reps: int = 5):
         p, z = probs[i], zs[i]
         h = Herk(L.FORM_C128, p["m"])
         v = h.new_vacc()
-        h.run([z.to(torch.complex128)], ldz=p["ldz"], nseg=p["nseg"],
+        h.run([z.to(torch.complex128)], ldz=zld(p), nseg=p["nseg"],
               seg_stride=p["seg_stride"], seg_len=p["seg_len"], vacc=v,
               accumulate=False, nslices=h.nslices(p["nseg"] * p["seg_len"]))
         m = p["m"]

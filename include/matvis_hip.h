@@ -53,7 +53,8 @@ typedef void *mv_stream_t;
  * src/matvis/core/beams.py:203-207). */
 int mv_last_error(char *buf, size_t len);
 
-/* Library ABI version (incremented on any signature change).  No reference
+/* Library ABI version (incremented on any change of a signature or of an
+ * argument's meaning; 3: split terms are K-tile-major).  No reference
  * counterpart: lets a binding check it loaded a matching library. */
 int mv_abi_version(void);
 
@@ -226,13 +227,16 @@ int mv_zmat_parts(const void *sqrt_flux, const void *beam,
  * compaction's global source ids) is non-NULL, else as flux[s].
  * beam: (nbeam, nfeed, nax, ld_beam) complex (beam_is_f64).
  * form: MV_FORM_*; z0..z2 are the output arrays: z0 the complex matrix for
- * plain forms; for split forms the hi/mid/lo term arrays (nterms of them),
- * 32 bits per complex element: each 16-byte chunk of 4 consecutive columns
- * holds the 4 real parts (16 bits each) and then the 4 imaginary parts
- * (split forms need ldz and seg_stride multiples of 4, so that rows and
- * segments start on chunks).  Layout of every output: row r at r*ldz
- * (units of complex elements), axis segment ax at ax*seg_stride.
- * Columns s in [n, ceil(n/pad)*pad) are zero-filled (pad = 32).
+ * plain forms, row r at r*ldz (complex elements), axis segment ax at
+ * ax*seg_stride.  For split forms the hi/mid/lo term arrays (nterms of
+ * them), 32 bits per complex element, K-tile-major: the columns of all
+ * segments (column s of segment ax is column ax*seg_stride + s) are cut
+ * into K tiles of 16; tile q holds ldz rows (ldz >= nant*nfeed) of 64
+ * bytes, row r at byte (q*ldz + r)*64; in a row, each 16-byte chunk of 4
+ * consecutive columns holds the 4 real parts (16 bits each) and then the 4
+ * imaginary parts.  Split forms need seg_stride a multiple of 16, so that
+ * segments start on K tiles.  Columns s in [n, ceil(n/pad)*pad) are
+ * zero-filled (pad = 32).
  * scale: multiplies Z before the split (a power of two; 1 for plain). */
 int mv_zmat_fused(const double *antpos, int nant, double freq,
                   const double *crd, int64_t ld_crd, const double *flux,
@@ -245,8 +249,9 @@ int mv_zmat_fused(const double *antpos, int nant, double freq,
 
 /* Split a complex matrix (c128 or c64, layout as mv_zmat_fused output) into
  * nterms 16-bit terms of `form` (SPLIT_BF16 / SPLIT_F16), after multiplying
- * by `scale`, in the chunked term layout of mv_zmat_fused (ldt and, with
- * nseg > 1, tseg_stride multiples of 4).  Rows [0, rows), columns
+ * by `scale`, in the K-tile-major term layout of mv_zmat_fused (ldt rows a
+ * K tile, ldt >= rows; with nseg > 1, tseg_stride a multiple of 16).  Rows
+ * [0, rows), columns
  * [0, cols) of each of nseg segments; the slots of a segment's last chunk
  * past cols are written as zeros.
  * For Z built outside mv_zmat_fused, e.g. by the plugin chain's ZMatrixCalc
@@ -291,7 +296,9 @@ int mv_herk_slices(int form, int nterms, int nprod, int m, int nseg,
  * z0..z2: Z (plain) or its split terms (nterms, nprod significant products:
  * 3 = hi*hi + hi*lo + lo*hi, 4 adds lo*lo, 6 = the 3-term set up to order 2).
  * Z columns: nseg segments, each seg_len long (*seg_len_dev if non-NULL),
- * starting at seg*seg_stride; ldz is the row stride; both in complex units.
+ * starting at seg*seg_stride (complex units).  Plain forms: ldz is the row
+ * stride (complex units).  Split forms: terms K-tile-major as mv_zmat_fused
+ * writes them, ldz the rows of a K tile (>= m), seg_stride a multiple of 16.
  * Split forms read whole tiles of 16 columns: every segment must have room
  * for round_up(seg_len, 16) columns and hold zeros from its column count
  * (*seg_len_dev when given) up to the next multiple of 16, as
@@ -350,7 +357,7 @@ int mv_herk_reduce(int form, int nterms, int nprod, const void *z0,
  * and device addresses so that a binding can pack an array of them
  * (104 bytes each; mv_herk_problem_bytes() reports the size).  Fields as in
  * mv_herk: z = term arrays (z[2] = 0 unless 3 terms), ldz / seg_stride /
- * seg_len in complex units (seg_len the host bound), seg_len_dev = device
+ * seg_len as there (seg_len the host bound), seg_len_dev = device
  * int32 column count or 0, vacc = device complex128 (ldv x ldv), scale,
  * accumulate; nslices = slice capacity given to this problem's share of the
  * workspace (mv_herk_max_slices, or more); kchain = the longest K-slice in

@@ -13,7 +13,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "csrc" / "libmatvis_hip.so"
 
 #: the library's mv_abi_version() this binding is written for
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 MV_OK = 0
 MV_ERR_ARG = -1

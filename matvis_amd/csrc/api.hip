@@ -35,4 +35,4 @@ extern "C" int mv_last_error(char *buf, size_t len) {
   return mv::g_code;
 }
 
-extern "C" int mv_abi_version(void) { return 2; }
+extern "C" int mv_abi_version(void) { return 3; }

@@ -387,6 +387,14 @@ __device__ __forceinline__ double row_norm2(const u32x4 (*r)[2]) {
 // One HERK instance of a split form as the kernels read it.  `part` holds
 // `nslices` x ntiles partial tiles of TILE x TILE complex<float>, `dpart`
 // the FP64 row norms of the diagonal tiles (nslices x ntiles x TILE).
+//
+// Split terms are stored K-tile-major: K tile q (columns 16 q .. 16 q + 15
+// of the concatenated segments, seg_stride a multiple of 16) is `ldz` rows
+// of 16 words, row r at word (q ldz + r) 16.  The rows one copy
+// instruction reads (16 consecutive rows of one K tile) are 1 KB of
+// contiguous memory and a workgroup's panel of a K tile is one run of
+// 128 x 64 bytes, where a row-major layout puts every row's 64 bytes on its
+// own page.
 struct SplitProblem {
   const uint32_t *t[3];
   int64_t ldz, seg_stride, seg_len_host;
@@ -504,8 +512,8 @@ __device__ __forceinline__ void split_item(const SplitProblem &P, int slice,
     const int c = (lane & 3) ^ swz(r);
     const int ra = bi * TILE + r, rb = bj * TILE + r;
     const int ga = ra < P.m ? ra : P.m - 1, gb = rb < P.m ? rb : P.m - 1;
-    coff_a[u] = (int64_t)ga * P.ldz + 4 * c;
-    coff_b[u] = (int64_t)gb * P.ldz + 4 * c;
+    coff_a[u] = (int64_t)ga * kSK + 4 * c;
+    coff_b[u] = (int64_t)gb * kSK + 4 * c;
   }
   // diagonal sums: piece pi covers half lw of row lrow (8 words)
   int lrow[S::kPairs], lw[S::kPairs];
@@ -531,8 +539,9 @@ __device__ __forceinline__ void split_item(const SplitProblem &P, int slice,
   // whole K tiles: columns [seg_len, round_up(seg_len, 16)) of every
   // segment are zero by contract
   int sg = kt0 / (kps > 0 ? kps : 1), kc = kt0 - sg * kps;
+  const int64_t seg_tiles = P.seg_stride / kSK;
   auto copy = [&](SplitStage<NT, TILE> &D) {
-    const int64_t k0 = sg * P.seg_stride + kc * kSK;
+    const int64_t k0 = (sg * seg_tiles + kc) * P.ldz * kSK;
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
@@ -1226,7 +1235,9 @@ int herk_run(int phases, int form, int nterms, int nprod, const void *z0,
   MV_REQUIRE(nseg == 1 || seg_stride >= seg_len,
              "mv_herk: segments overlap (seg_stride %lld < seg_len %lld)",
              (long long)seg_stride, (long long)seg_len);
-  MV_REQUIRE(ldz >= (int64_t)(nseg - 1) * seg_stride + seg_len,
+  MV_REQUIRE(ldz >= (form == MV_FORM_C128 || form == MV_FORM_C64
+                         ? (int64_t)(nseg - 1) * seg_stride + seg_len
+                         : (int64_t)m),
              "mv_herk: ldz %lld too small", (long long)ldz);
   MV_REQUIRE(cdiv(seg_len, (int64_t)16) * nseg < (int64_t)INT32_MAX / 2,
              "mv_herk: K too large");
@@ -1284,17 +1295,14 @@ int herk_run(int phases, int form, int nterms, int nprod, const void *z0,
     return check_launch("mv_herk/reduce");
   }
   MV_REQUIRE(z1 && (nterms < 3 || z2), "mv_herk: missing term arrays");
-  MV_REQUIRE(ldz % 4 == 0 && (nseg == 1 || seg_stride % 4 == 0) &&
-                 ((uintptr_t)z0 | (uintptr_t)z1 | (uintptr_t)(z2 ? z2 : z0)) %
-                         16 == 0,
-             "mv_herk: split terms need 16-byte aligned rows (ldz and "
-             "seg_stride multiples of 4)");
+  MV_REQUIRE(((uintptr_t)z0 | (uintptr_t)z1 | (uintptr_t)(z2 ? z2 : z0)) %
+                     16 == 0,
+             "mv_herk: split terms must be 16-byte aligned");
   const int64_t padded = cdiv(seg_len, (int64_t)kSK) * kSK;
-  MV_REQUIRE((nseg == 1 || seg_stride >= padded) &&
-                 ldz >= (int64_t)(nseg - 1) * seg_stride + padded,
-             "mv_herk: split terms need each segment padded to a multiple "
-             "of %d columns (seg_stride %lld, ldz %lld, seg_len %lld)",
-             kSK, (long long)seg_stride, (long long)ldz, (long long)seg_len);
+  MV_REQUIRE(nseg == 1 || (seg_stride % kSK == 0 && seg_stride >= padded),
+             "mv_herk: split terms need segments of whole %d-column K tiles "
+             "(seg_stride %lld, seg_len %lld)",
+             kSK, (long long)seg_stride, (long long)seg_len);
   const bool f16 = form == MV_FORM_SPLIT_F16;
   double *dpart = (double *)((char *)workspace + split_partial_bytes(m, nslices));
   SplitProblem P;
@@ -1528,15 +1536,14 @@ extern "C" int mv_herk_grouped(int form, int nterms, int nprod, int tile,
                "mv_herk_grouped: problem %d: null pointer", i);
     if (split) {
       const int64_t padded = cdiv(q.seg_len, (int64_t)kSK) * kSK;
-      MV_REQUIRE(q.ldz % 4 == 0 && (q.nseg == 1 || q.seg_stride % 4 == 0) &&
-                     ((q.z[0] | q.z[1] | (q.z[2] ? q.z[2] : q.z[0])) % 16) ==
-                         0,
-                 "mv_herk_grouped: problem %d: terms need 16-byte aligned "
-                 "rows", i);
-      MV_REQUIRE((q.nseg == 1 || q.seg_stride >= padded) &&
-                     q.ldz >= (int64_t)(q.nseg - 1) * q.seg_stride + padded,
-                 "mv_herk_grouped: problem %d: segments must hold "
-                 "round_up(seg_len, %d) columns", i, kSK);
+      MV_REQUIRE(((q.z[0] | q.z[1] | (q.z[2] ? q.z[2] : q.z[0])) % 16) == 0,
+                 "mv_herk_grouped: problem %d: terms must be 16-byte "
+                 "aligned", i);
+      MV_REQUIRE((q.nseg == 1 ||
+                  (q.seg_stride % kSK == 0 && q.seg_stride >= padded)) &&
+                     q.ldz >= q.m,
+                 "mv_herk_grouped: problem %d: split terms need segments of "
+                 "whole %d-column K tiles and ldz >= m", i, kSK);
     } else {
       MV_REQUIRE((q.nseg == 1 || q.seg_stride >= q.seg_len) &&
                      q.ldz >= (int64_t)(q.nseg - 1) * q.seg_stride + q.seg_len,

@@ -12,8 +12,8 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kZPad = 32;  // Z columns are zero-padded to a multiple of this
-// longest Z or term row (elements): a lane's column within a row is a 32-bit
-// offset from the row start
+// longest Z row, and most term columns (elements): a lane's column is a
+// 32-bit index
 constexpr int64_t kMaxRow = int64_t(1) << 30;
 
 __device__ __forceinline__ double reduced_turns(const double *__restrict__ ap,
@@ -135,24 +135,28 @@ __device__ __forceinline__ void split_terms(float re, float im, uint32_t *t) {
 // the row (and segment) start, the same in every lane of the wave; `col` the
 // lane's column within it, below 2^30 (the entry points check the row length).
 template <int FORM, int NT> struct ZWriter;
+// put(zs, r, ld, col, v) stores element (r, col).  Plain forms: row-major,
+// row stride ld elements.
 template <> struct ZWriter<MV_FORM_C128, 1> {
   typedef double T;
-  __device__ static void put(void *const *zs, int64_t row, uint32_t col,
-                             cplx<double> v) {
-    ((cplx<double> *)zs[0] + row)[col] = v;
+  __device__ static void put(void *const *zs, int64_t r, int64_t ld,
+                             uint32_t col, cplx<double> v) {
+    ((cplx<double> *)zs[0] + r * ld)[col] = v;
   }
 };
 template <> struct ZWriter<MV_FORM_C64, 1> {
   typedef float T;
-  __device__ static void put(void *const *zs, int64_t row, uint32_t col,
-                             cplx<float> v) {
-    ((cplx<float> *)zs[0] + row)[col] = v;
+  __device__ static void put(void *const *zs, int64_t r, int64_t ld,
+                             uint32_t col, cplx<float> v) {
+    ((cplx<float> *)zs[0] + r * ld)[col] = v;
   }
 };
-// Split forms: per term, 16-byte chunks of 4 consecutive columns, the real
-// parts of the 4 columns (16 bits each) in the chunk's first 8 bytes and
-// their imaginary parts in its last 8.  Column c of a row is slot c % 4 of
-// chunk c / 4 (rows and segments start on chunk boundaries).  Lanes 4q ..
+// Split forms: per term, K tiles of 16 columns, each `ld` rows of 64 bytes
+// (K-tile-major: row r of tile q at word (q ld + r) 16, the layout the HERK
+// copies from).  A row of a tile is 4 16-byte chunks of 4 consecutive
+// columns, the real parts of the 4 columns (16 bits each) in the chunk's
+// first 8 bytes and their imaginary parts in its last 8.  Column c is slot
+// c % 4 of chunk (c % 16) / 4 of tile c / 16.  Lanes 4q ..
 // 4q + 3 hold the 4 columns of one chunk.  Each lane packs its (re, im) term
 // into one word, takes its xor-1 partner's word (a DPP quad permutation
 // within the quad) and stores one whole word: even slots j the real parts
@@ -160,15 +164,16 @@ template <> struct ZWriter<MV_FORM_C64, 1> {
 // columns j - 1, j (word 2 + j / 2), formed by one byte permutation of the
 // two words.  Every lane of a quad must call put() together.
 template <bool F16, int NT>
-__device__ __forceinline__ void put_split(void *const *zs, int64_t row,
-                                          uint32_t col, cplx<float> v) {
+__device__ __forceinline__ void put_split(void *const *zs, int64_t r,
+                                          int64_t ld, uint32_t col,
+                                          cplx<float> v) {
   uint32_t t[NT];
   split_terms<F16, NT>(v.re, v.im, t);
   const uint32_t j = col & 3u;
   const bool odd = j & 1u;
-  // byte offset of the lane's word within the row: 32 bits, so that the
-  // store takes the row start as a scalar base
-  const uint32_t wb = (col - j + (odd ? 2u : 0u) + (j >> 1)) * 4u;
+  // word of the tile row, then the byte offset of the lane's word in it
+  const int64_t row = ((int64_t)(col >> 4) * ld + r) * 16;
+  const uint32_t wb = ((col & 15u) - j + (odd ? 2u : 0u) + (j >> 1)) * 4u;
   // bytes of {partner, mine}: even lanes take my low half then the
   // partner's low half; odd lanes the partner's high half then my high half
   const uint32_t sel = odd ? 0x03020706u : 0x05040100u;
@@ -183,16 +188,16 @@ __device__ __forceinline__ void put_split(void *const *zs, int64_t row,
 }
 template <int NT> struct ZWriter<MV_FORM_SPLIT_BF16, NT> {
   typedef float T;
-  __device__ static void put(void *const *zs, int64_t row, uint32_t col,
-                             cplx<float> v) {
-    put_split<false, NT>(zs, row, col, v);
+  __device__ static void put(void *const *zs, int64_t r, int64_t ld,
+                             uint32_t col, cplx<float> v) {
+    put_split<false, NT>(zs, r, ld, col, v);
   }
 };
 template <int NT> struct ZWriter<MV_FORM_SPLIT_F16, NT> {
   typedef float T;
-  __device__ static void put(void *const *zs, int64_t row, uint32_t col,
-                             cplx<float> v) {
-    put_split<true, NT>(zs, row, col, v);
+  __device__ static void put(void *const *zs, int64_t r, int64_t ld,
+                             uint32_t col, cplx<float> v) {
+    put_split<true, NT>(zs, r, ld, col, v);
   }
 };
 
@@ -260,7 +265,7 @@ __global__ __launch_bounds__(kThreads) void zmat_fused_kernel(
     for (int fd = 0; fd < nf; ++fd)
 #pragma unroll
       for (int ax = 0; ax < na; ++ax)
-        ZWriter<FORM, NT>::put(zo.z, ((int64_t)a * nf + fd) * ldz,
+        ZWriter<FORM, NT>::put(zo.z, (int64_t)a * nf + fd, ldz,
                                (uint32_t)(ax * seg_stride + s),
                                out[fd * na + ax]);
   }
@@ -281,7 +286,7 @@ __global__ __launch_bounds__(kThreads) void split_kernel(
     const cplx<TI> v = z[r * ldz + sg * seg_stride + c];
     w = cplx<float>{(float)v.re * scale, (float)v.im * scale};
   }
-  ZWriter<FORM, NT>::put(to.z, r * ldt + sg * tseg_stride, (uint32_t)c, w);
+  ZWriter<FORM, NT>::put(to.z, r, ldt, (uint32_t)(sg * tseg_stride + c), w);
 }
 
 template <int FORM, int NT>
@@ -384,14 +389,15 @@ extern "C" int mv_zmat_fused(const double *antpos, int nant, double freq,
              "mv_zmat_fused: seg_stride (%lld) must cover count rounded up "
              "to %d",
              (long long)seg_stride, kZPad);
-  MV_REQUIRE(ldz >= (int64_t)nax * seg_stride,
-             "mv_zmat_fused: ldz must be >= nax * seg_stride");
-  MV_REQUIRE(ldz <= kMaxRow, "mv_zmat_fused: ldz must be at most 2^30");
-  MV_REQUIRE((form != MV_FORM_SPLIT_BF16 && form != MV_FORM_SPLIT_F16) ||
-                 (ldz % 4 == 0 && seg_stride % 4 == 0),
-             "mv_zmat_fused: split terms need ldz and seg_stride multiples "
-             "of 4 (4-column chunks)");
   const bool plain = form == MV_FORM_C128 || form == MV_FORM_C64;
+  // plain: row-major rows of ldz elements; split: K tiles of ldz rows
+  MV_REQUIRE(plain ? ldz >= (int64_t)nax * seg_stride
+                   : ldz >= (int64_t)nant * nfeed && seg_stride % 16 == 0,
+             plain ? "mv_zmat_fused: ldz must be >= nax * seg_stride"
+                   : "mv_zmat_fused: split terms need ldz >= nant * nfeed "
+                     "rows a K tile and seg_stride a multiple of 16");
+  MV_REQUIRE(ldz <= kMaxRow && (int64_t)nax * seg_stride <= kMaxRow,
+             "mv_zmat_fused: ldz and nax * seg_stride must be at most 2^30");
   MV_REQUIRE(plain || form == MV_FORM_SPLIT_BF16 || form == MV_FORM_SPLIT_F16,
              "mv_zmat_fused: unknown form %d", form);
   MV_REQUIRE(plain ? nterms == 1 : (nterms == 2 || nterms == 3),
@@ -441,14 +447,16 @@ extern "C" int mv_split_z(const void *z, int z_is_f64, int64_t rows, int nseg,
   MV_REQUIRE(nterms == 2 || nterms == 3, "mv_split_z: nterms must be 2 or 3");
   MV_REQUIRE(nterms < 3 || (t1 && t2), "mv_split_z: missing term arrays");
   MV_REQUIRE(t1, "mv_split_z: missing term array 1");
+  // terms: K tiles of 16 columns, ldt rows each (ldt >= rows)
   MV_REQUIRE(rows >= 0 && nseg >= 1 && cols >= 0 && rows * nseg <= 65535 &&
                  seg_stride >= cols && ldz >= nseg * seg_stride &&
-                 tseg_stride >= cols && ldt >= nseg * tseg_stride,
+                 tseg_stride >= cols && ldt >= rows,
              "mv_split_z: bad sizes");
-  MV_REQUIRE(ldt <= kMaxRow, "mv_split_z: ldt must be at most 2^30");
-  MV_REQUIRE(ldt % 4 == 0 && (nseg == 1 || tseg_stride % 4 == 0),
-             "mv_split_z: term rows and segments must start on 4-column "
-             "chunks (ldt and tseg_stride multiples of 4)");
+  MV_REQUIRE(ldt <= kMaxRow && nseg * tseg_stride <= kMaxRow,
+             "mv_split_z: ldt and nseg * tseg_stride must be at most 2^30");
+  MV_REQUIRE(nseg == 1 || tseg_stride % 16 == 0,
+             "mv_split_z: term segments must start on 16-column K tiles "
+             "(tseg_stride a multiple of 16)");
   if (rows == 0 || cols == 0) return MV_OK;
   dim3 g((unsigned)cdiv(cols, kThreads), (unsigned)(rows * nseg));
   hipStream_t st = as_stream(stream);

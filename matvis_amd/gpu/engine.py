@@ -30,7 +30,7 @@ from .._lib import FORM_C128, FORM_SPLIT_F16, call, lib
 from .._utils import log_progress
 from .._utils import logger as _progress_log
 from .beams import DeviceBeamSet
-from .herk import Herk, cdiv, expand, is_split
+from .herk import Herk, cdiv, expand, is_split, split_terms_alloc
 from .matprod import f16_scale
 
 Z_PAD = 32  # mv_zmat_fused pads Z columns to a multiple of this
@@ -151,13 +151,15 @@ class VisibilityEngine:
         nA = nb if shared else 1
         self.A = [D.zeros((self.nbeam, self.nfeed, self.nax, seg),
                           self.beam_ctype) for _ in range(nA)]
-        # Z: (m, nax, seg) complex, or nterms arrays of 16-bit terms (32 bits
-        # a complex element, 16-byte chunks of 4 columns: re x 4, im x 4)
-        self.ldz = self.nax * seg
+        # Z: (m, nax, seg) complex (ldz the row stride), or nterms arrays of
+        # 16-bit terms, K-tile-major (ldz the rows of a 16-column K tile;
+        # 32 bits a complex element, 16-byte chunks of 4 columns: re x 4,
+        # im x 4)
         if is_split(self.form):
-            self.z = [D.zeros((self.m, self.ldz), np.int32)
-                      for _ in range(self.nterms)]
+            self.ldz = self.m
+            self.z = split_terms_alloc(self.nterms, self.m, self.nax * seg)
         else:
+            self.ldz = self.nax * seg
             self.z = [D.zeros((self.m, self.ldz), self.beam_ctype)]
         self.herk = Herk(self.form, self.m, self.nterms, self.nprod)
         self.vacc = self.herk.new_vacc()

@@ -40,6 +40,19 @@ def cdiv(a: int, b: int) -> int:
     return -(-a // b)
 
 
+def split_terms_alloc(nterms: int, m: int, ncols: int) -> list:
+    """Zeroed device term arrays of a split form for an m-row Z of `ncols`
+    columns (a multiple of 16, segments included), K-tile-major as
+    ``mv_zmat_fused`` and ``mv_split_z`` write them and the split HERK reads
+    them: K tile q (16 columns) is m rows of 16 int32 words, so the arrays
+    are (ncols / 16, m, 16) and the ``ldz`` / ``ldt`` to pass is m."""
+    if ncols % K_TILE:
+        raise ValueError(f"split terms need whole {K_TILE}-column K tiles "
+                         f"(ncols={ncols})")
+    return [D.zeros((ncols // K_TILE, m, K_TILE), np.int32)
+            for _ in range(nterms)]
+
+
 def is_split(form: int) -> bool:
     return form in (FORM_SPLIT_BF16, FORM_SPLIT_F16)
 

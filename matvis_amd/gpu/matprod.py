@@ -20,7 +20,7 @@ import torch
 from .. import _device as D
 from .._lib import FORM_C64, FORM_C128, FORM_SPLIT_BF16, FORM_SPLIT_F16, call
 from ..core.matprod import MatProd
-from .herk import Herk, expand, products_for
+from .herk import Herk, expand, products_for, split_terms_alloc
 
 
 class GPUMatMul(MatProd):
@@ -123,23 +123,23 @@ class GPUMatMulSplitBF16(GPUMatMul):
     def compute(self, z, out):
         z = self._check_z(z)
         k = z.shape[1]
-        # the HERK reads whole 16-column tiles: columns [k, ldt) must be zero
-        ldt = max(16, -(-k // 16) * 16)
-        if getattr(self, "_terms", None) is None or self._terms[0].shape[1] < ldt:
-            self._terms = [D.zeros((self.m, ldt), np.int32)
-                           for _ in range(self.nterms)]
-        else:
+        # the HERK reads whole 16-column K tiles: columns [k, ncols) of the
+        # last one must be zero
+        ncols = max(16, -(-k // 16) * 16)
+        if (getattr(self, "_terms", None) is None
+                or self._terms[0].shape[0] * 16 < ncols):
+            self._terms = split_terms_alloc(self.nterms, self.m, ncols)
+        elif k % 16:
             for term in self._terms:
-                term[:, k:ldt].zero_()
+                term[k // 16].zero_()
         t = self._terms + [None] * (3 - self.nterms)
         s = self._scale(z)
         call("mv_split_z", D.ptr(z), int(z.dtype == torch.complex128), self.m,
              1, k, z.stride(0), k, self.form, self.nterms, s, D.ptr(t[0]),
-             D.ptr(t[1]), D.ptr(t[2]), self._terms[0].stride(0),
-             self._terms[0].stride(0), D.stream_ptr())
-        self.herk.run(self._terms, ldz=self._terms[0].stride(0), nseg=1,
-                      seg_stride=self._terms[0].stride(0), seg_len=k,
-                      vacc=out, accumulate=False, scale=1.0 / (s * s))
+             D.ptr(t[1]), D.ptr(t[2]), self.m, ncols, D.stream_ptr())
+        self.herk.run(self._terms, ldz=self.m, nseg=1, seg_stride=ncols,
+                      seg_len=k, vacc=out, accumulate=False,
+                      scale=1.0 / (s * s))
         return out
 
 

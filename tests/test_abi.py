@@ -36,7 +36,7 @@ def test_ctypes_table_matches_header():
 
 
 def test_abi_version():
-    assert _lib.lib().mv_abi_version() == _lib.ABI_VERSION == 2
+    assert _lib.lib().mv_abi_version() == _lib.ABI_VERSION == 3
 
 
 def test_tiles_and_workspace():

@@ -28,14 +28,14 @@ def _problem(m, seg_len, nseg, n_dev, form, nterms, seed):
     from matvis_amd.gpu.matprod import f16_scale
     scale = f16_scale(float(np.abs(z).max())) if form == FORM_SPLIT_F16 else 1.0
     zd = D.to_dev(z)
-    terms = [D.zeros((m, nseg * seg), np.int32) for _ in range(nterms)]
+    from matvis_amd.gpu.herk import split_terms_alloc
+    terms = split_terms_alloc(nterms, m, nseg * seg)  # m rows a K tile
     t = terms + [None] * (3 - nterms)
     call("mv_split_z", D.ptr(zd), 0, m, nseg, seg_len, nseg * seg, seg, form,
-         nterms, scale, D.ptr(t[0]), D.ptr(t[1]), D.ptr(t[2]), nseg * seg,
+         nterms, scale, D.ptr(t[0]), D.ptr(t[1]), D.ptr(t[2]), m,
          seg, D.stream_ptr())
-    from matvis_amd.gpu.herk import Herk
     ldv = -(-m // 128) * 128
-    return dict(terms=terms, m=m, ldz=nseg * seg, nseg=nseg, seg_stride=seg,
+    return dict(terms=terms, m=m, ldz=m, nseg=nseg, seg_stride=seg,
                 seg_len=seg_len,
                 seg_len_dev=None if n_dev is None else D.to_dev(
                     np.array([n_dev], np.int32)),

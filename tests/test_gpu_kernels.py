@@ -321,15 +321,25 @@ def test_herk_segments_and_device_length():
 
 
 def _split_terms(z, form, nterms, scale=1.0):
+    """Split terms of z (K-tile-major, ldt = m rows a K tile) and their
+    column count, zero-padded to whole 16-column K tiles."""
+    from matvis_amd.gpu.herk import split_terms_alloc
     m, k = z.shape
-    ldt = max(16, -(-k // 16) * 16)  # zero-padded to whole 16-column tiles
-    ts = [D.zeros((m, ldt), np.int32) for _ in range(nterms)]
+    ncols = max(16, -(-k // 16) * 16)
+    ts = split_terms_alloc(nterms, m, ncols)
     t = ts + [None] * (3 - nterms)
     zd = _dev(z)
     call("mv_split_z", D.ptr(zd), int(z.dtype == np.complex128), m, 1, k,
-         k, k, form, nterms, scale, D.ptr(t[0]), D.ptr(t[1]), D.ptr(t[2]), ldt,
-         ldt, D.stream_ptr())
-    return ts, ldt
+         k, k, form, nterms, scale, D.ptr(t[0]), D.ptr(t[1]), D.ptr(t[2]), m,
+         ncols, D.stream_ptr())
+    return ts, ncols
+
+
+def _row_major(t):
+    """Words of a K-tile-major term array (ntiles, rows, 16) as the
+    row-major (rows, ntiles * 16) matrix they store."""
+    w = D.to_host(t).view(np.uint32)
+    return w.transpose(1, 0, 2).reshape(w.shape[1], -1)
 
 
 def _np_term_words(re, im, f16, nterms):
@@ -384,13 +394,13 @@ def test_split_z_terms_bit_exact(form, nterms, zdtype):
     z[1, 3] = 0
     z[2, 10] = -0.75 + 0.0j
     scale = 0.5
-    ts, ldt = _split_terms(z, form, nterms, scale)
+    ts, ncols = _split_terms(z, form, nterms, scale)
     re = z.real.astype(np.float32) * np.float32(scale)
     im = z.imag.astype(np.float32) * np.float32(scale)
     want = _np_term_words(re, im, form == FORM_SPLIT_F16, nterms)
     k4 = want[0].shape[1]
     for t, w in zip(ts, want):
-        got = D.to_host(t).view(np.uint32)
+        got = _row_major(t)
         assert np.array_equal(got[:, :k4], w)
         assert np.all(got[:, k4:] == 0)
 
@@ -420,29 +430,30 @@ def test_zmat_fused_split_is_the_split_of_its_complex64(polarized):
         bidx=_dev((np.arange(nant) % 2).astype(np.int32)),
         n=_dev(np.array([n], np.int32)))
 
-    def fused(form, nterms, outs):
+    def fused(form, nterms, outs, ldz):
         zp = [D.ptr(o) for o in outs] + [None] * (3 - len(outs))
         call("mv_zmat_fused", D.ptr(bufs["ant"]), nant, 150e6,
              D.ptr(bufs["crd"]), seg, D.ptr(bufs["flux"]), D.ptr(bufs["idx"]),
              D.ptr(bufs["beam"]), 0, seg, 2, D.ptr(bufs["bidx"]), nf, nf, 900,
              D.ptr(bufs["n"]), form, nterms, 1.0, zp[0], zp[1], zp[2],
-             nf * seg, seg, D.stream_ptr())
+             ldz, seg, D.stream_ptr())
 
     z64 = D.zeros((nant * nf, nf * seg), np.complex64)
-    fused(FORM_C64, 1, [z64])
+    fused(FORM_C64, 1, [z64], nf * seg)
     z = D.to_host(z64)
     npad = -(-n // 32) * 32
     for form, nterms in [(FORM_SPLIT_BF16, 2), (FORM_SPLIT_BF16, 3),
                          (FORM_SPLIT_F16, 2)]:
-        outs = [D.zeros((nant * nf, nf * seg), np.int32) + 0x5A5A5A5A
-                for _ in range(nterms)]
-        fused(form, nterms, outs)
+        # K-tile-major terms, nant * nf rows a K tile
+        outs = [D.zeros((nf * seg // 16, nant * nf, 16), np.int32)
+                + 0x5A5A5A5A for _ in range(nterms)]
+        fused(form, nterms, outs, nant * nf)
         for ax in range(nf):
             cols = slice(ax * seg, ax * seg + npad)
             want = _np_term_words(z[:, cols].real, z[:, cols].imag,
                                   form == FORM_SPLIT_F16, nterms)
             for o, w in zip(outs, want):
-                got = D.to_host(o).view(np.uint32)
+                got = _row_major(o)
                 assert np.array_equal(got[:, cols], w), (form, nterms, ax)
                 assert np.all(got[:, ax * seg + npad:(ax + 1) * seg]
                               == 0x5A5A5A5A)
@@ -465,11 +476,11 @@ def test_herk_split_accuracy(form, nterms, nprod, bound, m, k):
     z = (amp * np.exp(2j * np.pi * rng.uniform(size=(m, k)))).astype(np.complex64)
     from matvis_amd.gpu.matprod import f16_scale
     s = f16_scale(float(np.abs(z).max())) if form == FORM_SPLIT_F16 else 1.0
-    ts, ldt = _split_terms(z, form, nterms, s)
+    ts, ncols = _split_terms(z, form, nterms, s)
     # FP32 accumulation spans one K-slice; the slice sums are FP64
     from matvis_amd.gpu.herk import Herk
     cap = Herk(form, m, nterms, nprod).nslices(k)
-    h, v = _run_herk(form, ts, m, ldt, 1, ldt, k, nterms, nprod,
+    h, v = _run_herk(form, ts, m, m, 1, ncols, k, nterms, nprod,
                      nslices=cap, scale=1.0 / (s * s))
     got = _vis_full(h, v, m)
     z64 = z.astype(np.complex128)
@@ -559,15 +570,17 @@ def test_herk_phases_match_single_call(form, nterms, nprod):
     z = _herk_case(m, k, np.complex64, seed=5)
     if form == FORM_C128:
         zs, ld = [_dev(z.astype(np.complex128))], k
+        ldz = ld
     else:
         zs, ld = _split_terms(z, form, nterms)
+        ldz = m  # rows of a K tile
     h = Herk(form, m, nterms, nprod)
     v1, v2 = h.new_vacc(), h.new_vacc()
     cap = h.nslices(k)
-    h.run(zs, ldz=ld, nseg=1, seg_stride=ld, seg_len=k, vacc=v1,
+    h.run(zs, ldz=ldz, nseg=1, seg_stride=ld, seg_len=k, vacc=v1,
           accumulate=False, nslices=cap)
     marks = []
-    h.run(zs, ldz=ld, nseg=1, seg_stride=ld, seg_len=k, vacc=v2,
+    h.run(zs, ldz=ldz, nseg=1, seg_stride=ld, seg_len=k, vacc=v2,
           accumulate=False, nslices=cap, between=lambda s: marks.append(s))
     assert len(marks) == 1
     np.testing.assert_array_equal(_vis_full(h, v1, m), _vis_full(h, v2, m))

@@ -34,6 +34,7 @@ import matvis_amd._device as D
 import oracle as O
 from matvis_amd import synth
 from matvis_amd._lib import FORM_C128, FORM_SPLIT_BF16, call
+from matvis_amd.gpu.herk import split_terms_alloc
 
 pytestmark = pytest.mark.gpu
 
@@ -59,12 +60,12 @@ def _batch(ninst=256, seed=7):
                                     device="cuda", dtype=torch.float64)
         z = torch.polar(amp.expand(m, -1), ph).to(torch.complex64)
         z.view(m, nf, seg)[:, :, nup:] = 0
-        terms = [D.zeros((m, nf * seg), np.int32) for _ in range(2)]
+        terms = split_terms_alloc(2, m, nf * seg)  # m rows a K tile
         call("mv_split_z", D.ptr(z), 0, m, nf, nup, nf * seg, seg,
              FORM_SPLIT_BF16, 2, 1.0, D.ptr(terms[0]), D.ptr(terms[1]), None,
-             nf * seg, seg, D.stream_ptr())
+             m, seg, D.stream_ptr())
         ldv = -(-m // 128) * 128
-        probs.append(dict(terms=terms, m=m, ldz=nf * seg, nseg=nf,
+        probs.append(dict(terms=terms, m=m, ldz=m, nseg=nf,
                           seg_stride=seg, seg_len=nup,
                           vacc=D.zeros((ldv, ldv), np.complex128), ldv=ldv,
                           scale=1.0, accumulate=False))
@@ -113,7 +114,8 @@ def test_batched_workload_against_fp64():
     for i, (p, z) in enumerate(zip(probs, zs)):
         h = Herk(FORM_C128, p["m"])
         v = h.new_vacc()
-        h.run([z.to(torch.complex128)], ldz=p["ldz"], nseg=p["nseg"],
+        h.run([z.to(torch.complex128)], ldz=p["nseg"] * p["seg_stride"],
+              nseg=p["nseg"],
               seg_stride=p["seg_stride"], seg_len=p["seg_len"], vacc=v,
               accumulate=False, nslices=h.nslices(p["nseg"] * p["seg_len"]))
         e = _upper_nerr(p["vacc"], v, p["m"])

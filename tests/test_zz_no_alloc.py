@@ -13,7 +13,7 @@ import torch
 
 import matvis_amd._device as D
 from matvis_amd._lib import FORM_C128, FORM_SPLIT_BF16, HipError, call, lib
-from matvis_amd.gpu.herk import Herk
+from matvis_amd.gpu.herk import Herk, split_terms_alloc
 
 pytestmark = pytest.mark.gpu
 
@@ -30,9 +30,9 @@ def _z(m, nseg, nup, nvalid, seed, dtype):
 
 
 def _terms(z, m, nseg, nup, seg):
-    ts = [D.zeros((m, nseg * seg), np.int32) for _ in range(2)]
+    ts = split_terms_alloc(2, m, nseg * seg)  # K-tile-major, m rows a tile
     call("mv_split_z", D.ptr(z), 0, m, nseg, nup, nseg * seg, seg,
-         FORM_SPLIT_BF16, 2, 1.0, D.ptr(ts[0]), D.ptr(ts[1]), None, nseg * seg,
+         FORM_SPLIT_BF16, 2, 1.0, D.ptr(ts[0]), D.ptr(ts[1]), None, m,
          seg, D.stream_ptr())
     return ts
 
@@ -51,11 +51,12 @@ def _both(form, terms, m, nseg, seg, nup, ndev=None):
     h = Herk(form, m, nt, npd)
     dev = D.to_dev(np.array([ndev], np.int32)) if ndev is not None else None
     p = [D.ptr(t) for t in terms] + [None] * (3 - len(terms))
-    want = h.run(terms, ldz=nseg * seg, nseg=nseg, seg_stride=seg,
+    ldz = m if form == FORM_SPLIT_BF16 else nseg * seg
+    want = h.run(terms, ldz=ldz, nseg=nseg, seg_stride=seg,
                  seg_len=nup, seg_len_dev=dev, vacc=h.new_vacc(),
                  accumulate=False)
     got = h.new_vacc()
-    call("mv_herk_noalloc", form, nt, npd, p[0], p[1], p[2], m, nseg * seg,
+    call("mv_herk_noalloc", form, nt, npd, p[0], p[1], p[2], m, ldz,
          nseg, seg, nup, D.ptr(dev), 1.0, D.ptr(got), h.ldv, 0,
          D.stream_ptr())
     torch.cuda.synchronize()
